@@ -1,0 +1,71 @@
+/*
+ * oracle.h -- CPU restatement of the reference isosurface hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product (miniisosurface_amd/, include/) links,
+ * loads or calls this code. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg use it, and there only as the checker / reported CPU baseline.
+ *
+ * What it restates (reference paths relative to the miniIsosurface root):
+ *   ora_fe_*  : flyingEdges/serial/FlyingEdgesAlgorithm.cpp (pass1 :23-66, pass2 :72-190,
+ *               pass3 :196-269, pass4 :275-497, helpers :515-647) and, when nthreads > 1, the
+ *               row-parallel structure of flyingEdges/openmp/FlyingEdgesAlgorithm.cpp
+ *               (pass1 :24-74, pass2 :80-203, blocked scans :209-332, pass4 :338-565).
+ *               Geometry helpers: flyingEdges/util/Image3D.cpp:20-188.
+ *   ora_mc_*  : marchingCubes/serial/main.cpp:42-192 (first-visit vertex numbering through a
+ *               map from global edge id, marchingCubes/util/Image3D.cpp:168-256) with
+ *               findCaseId / interpolate from marchingCubes/util/util.h:24-54.
+ *
+ * Pinning: every function here is checked in tests/ against fixtures written by the compiled
+ * reference (tests/golden/), against oracle/_ref where it is built, and against closed forms
+ * (counts from the sign field, vertex order rules, FP64 interpolation bounds).
+ */
+#ifndef MINIISO_ORACLE_H
+#define MINIISO_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Marching-cubes case tables, decoded from the per-case edge strings in tables.c. */
+extern unsigned char ora_num_tris[256];
+extern unsigned char ora_is_cut[256][12];
+extern signed char   ora_case_tris[256][16];   /* -1 terminated */
+extern const unsigned char ora_edge_vertices[12][2];
+void ora_tables_init(void);
+
+typedef struct ora_fe ora_fe;
+
+/* Runs all four passes. nthreads <= 1 runs the serial restatement, otherwise the OpenMP one. */
+ora_fe* ora_fe_run(const float* data, int64_t nx, int64_t ny, int64_t nz,
+                   const float spacing[3], const float origin[3], float iso, int nthreads);
+void    ora_fe_sizes(const ora_fe* h, uint64_t* npts, uint64_t* ntris);
+/* pass wall times in seconds: [ctor, pass1, pass2, pass3, pass4, total] */
+void    ora_fe_times(const ora_fe* h, double out[6]);
+/* Copies the mesh. Any pointer may be NULL. */
+void    ora_fe_mesh(const ora_fe* h, float* pts, float* nrm, uint64_t* tris);
+/* Per grid row (ny*nz rows): xl, xr and the x/y/z counts before the scan (pass 2 output),
+ * per cell row ((ny-1)*(nz-1)): triangle count before the scan. Any pointer may be NULL. */
+void    ora_fe_rows(const ora_fe* h, uint64_t* xl, uint64_t* xr, uint64_t* xcnt, uint64_t* ycnt,
+                    uint64_t* zcnt, uint64_t* tcnt);
+/* Per grid row offsets after pass 3 (xstart, ystart, zstart) and per cell row triangle offsets. */
+void    ora_fe_offsets(const ora_fe* h, uint64_t* xstart, uint64_t* ystart, uint64_t* zstart,
+                       uint64_t* toff);
+/* cubeCases ((nx-1)*(ny-1)*(nz-1) bytes, 0 outside the trimmed range) and edgeCases
+ * ((nx-1)*ny*nz bytes). Either pointer may be NULL. */
+void    ora_fe_cases(const ora_fe* h, unsigned char* cube_cases, unsigned char* edge_cases);
+void    ora_fe_free(ora_fe* h);
+
+typedef struct ora_mc ora_mc;
+ora_mc* ora_mc_run(const float* data, int64_t nx, int64_t ny, int64_t nz,
+                   const float spacing[3], const float origin[3], float iso);
+void    ora_mc_sizes(const ora_mc* h, uint64_t* npts, uint64_t* ntris, double* seconds);
+void    ora_mc_mesh(const ora_mc* h, float* pts, float* nrm, uint64_t* tris);
+void    ora_mc_free(ora_mc* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
