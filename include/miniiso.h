@@ -1,0 +1,120 @@
+/*
+ * miniiso.h -- C ABI of the MI355X isosurface engine (libminiiso.so).
+ *
+ * Plain C types only: pointers, sizes, status codes. No exception crosses this boundary; every
+ * call returns MISO_OK (0) or a negative status, and miso_last_error() describes the last failure
+ * on the calling thread. One handle is used from one host thread at a time; each handle owns one
+ * HIP stream (or borrows the caller's).
+ *
+ * Reference interfaces these entry points replace (paths relative to the miniIsosurface root):
+ *   FlyingEdgesAlgorithm(Image3D const&, scalar_t const&)   flyingEdges/serial/FlyingEdgesAlgorithm.h:29-39
+ *   void pass1(); pass2(); pass3(); pass4();                 flyingEdges/serial/FlyingEdgesAlgorithm.h:41-47
+ *   util::TriangleMesh moveOutput();                         flyingEdges/serial/FlyingEdgesAlgorithm.h:49
+ *   (the OpenMP class has the same interface: flyingEdges/openmp/FlyingEdgesAlgorithm.h:29-51)
+ *   template<T> TriangleMesh<T> MarchingCubes(Image3D<T> const&, T const&)
+ *                                                            marchingCubes/serial/main.cpp:42-44
+ *   MarchingCubes(image, isoval, nSectionsX, nSectionsY, nSectionsZ)
+ *                                                            marchingCubes/openmp/main.cpp:159-162
+ */
+#ifndef MINIISO_H
+#define MINIISO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MISO_OK 0
+#define MISO_ERR_HIP (-1)       /* a HIP runtime call failed */
+#define MISO_ERR_ALLOC (-2)     /* device or host allocation failed */
+#define MISO_ERR_ARG (-3)       /* invalid argument */
+#define MISO_ERR_STATE (-4)     /* call out of order (e.g. pass 3 before pass 2) */
+#define MISO_ERR_BUDGET (-5)    /* request exceeds the admission budget */
+
+/* An image volume: data[i + nx*(j + ny*k)], x fastest, as util::Image3D stores it
+ * (flyingEdges/util/Image3D.h:31-37, Image3D.cpp:110-114). */
+typedef struct miso_volume {
+    const float* data;       /* host pointer (on_device == 0) or device pointer (on_device == 1) */
+    int64_t nx, ny, nz;
+    float spacing[3];
+    float origin[3];         /* the reference's zeroPos */
+    int32_t on_device;
+} miso_volume;
+
+/* ---------------------------------------------------------------- Flying Edges */
+typedef struct miso_fe miso_fe;
+
+/* FlyingEdgesAlgorithm(image, isoval) (serial/FlyingEdgesAlgorithm.h:29-39). A host volume is
+ * copied to the device here; a device volume is borrowed and must outlive the handle.
+ * `stream` is a hipStream_t to run on, or NULL for a stream owned by the handle. */
+int miso_fe_create(const miso_volume* vol, float isoval, void* stream, miso_fe** out);
+
+/* Re-targets an existing handle to another isovalue without reallocating (passes restart at 1). */
+int miso_fe_set_isoval(miso_fe* h, float isoval);
+
+/* pass1()..pass4() (serial/FlyingEdgesAlgorithm.h:41-47); must be called in order 1, 2, 3, 4.
+ * Each call waits for its kernels to finish, so host timers around it measure the pass. */
+int miso_fe_pass(miso_fe* h, int pass);
+
+/* All four passes with a single host synchronisation (after pass 3, to size the output). */
+int miso_fe_run(miso_fe* h);
+
+/* Vertex and triangle counts; valid after pass 3 (the reference allocates its output there). */
+int miso_fe_counts(const miso_fe* h, uint64_t* nverts, uint64_t* ntris);
+
+/* moveOutput() (serial/FlyingEdgesAlgorithm.h:49): copies the mesh to host arrays of
+ * nverts*3 floats (points, normals) and ntris*3 indices of 8 bytes (the reference's size_t).
+ * Any pointer may be NULL to skip that array. */
+int miso_fe_copy_out(miso_fe* h, float* points, float* normals, uint64_t* triangles);
+
+/* Device-resident output (owned by the handle until the next run or destroy). Triangle indices
+ * are 32-bit when nverts < 2^32 (index_bits = 32), else 64-bit. */
+int miso_fe_device_output(miso_fe* h, const float** points, const float** normals,
+                          const void** triangles, int* index_bits);
+
+/* Forces 64-bit triangle indices (index_bits 64) or the default (32 when they fit). */
+int miso_fe_set_index_bits(miso_fe* h, int index_bits);
+
+/* Device time of the last run's passes in milliseconds: [pass1, pass2, pass3, pass4]. */
+int miso_fe_pass_times(const miso_fe* h, float ms[4]);
+
+/* Per grid row (ny*nz entries, r = j + ny*k): xl, xr (pass 1 trims), the cut x/y/z edge counts
+ * (pass 2, before the scan) and the vertex offset (after pass 3); per cell row, stored at the
+ * grid-row index of its (j,k) with j < ny-1 and k < nz-1: triangle count and offset. Arrays may
+ * be NULL. Valid after pass 3. */
+int miso_fe_rows(const miso_fe* h, uint32_t* xl, uint32_t* xr, uint32_t* xcount,
+                 uint32_t* ycount, uint32_t* zcount, uint32_t* tcount, uint64_t* voffset,
+                 uint64_t* toffset);
+
+/* The reference's cubeCases after pass 2 ((nx-1)*(ny-1)*(nz-1) bytes, x fastest): the
+ * marching-cubes case inside each cell row's trimmed range [xl, xr) and 0 elsewhere
+ * (FlyingEdgesAlgorithm.cpp:106-113). With `all_cells` nonzero, the true case of every cell. */
+int miso_fe_cube_cases(miso_fe* h, uint8_t* cases, int all_cells);
+
+void miso_fe_destroy(miso_fe* h);
+
+/* ---------------------------------------------------------------- Marching Cubes */
+typedef struct miso_mc miso_mc;
+
+/* MarchingCubes(image, isoval) (marchingCubes/serial/main.cpp:42-192): same triangle order as
+ * Flying Edges; vertices numbered in the serial reference's first-visit order. */
+int miso_mc_create(const miso_volume* vol, float isoval, void* stream, miso_mc** out);
+int miso_mc_run(miso_mc* h);
+int miso_mc_counts(const miso_mc* h, uint64_t* nverts, uint64_t* ntris);
+int miso_mc_copy_out(miso_mc* h, float* points, float* normals, uint64_t* triangles);
+int miso_mc_device_output(miso_mc* h, const float** points, const float** normals,
+                          const void** triangles, int* index_bits);
+int miso_mc_time_ms(const miso_mc* h, float* ms);
+void miso_mc_destroy(miso_mc* h);
+
+/* ---------------------------------------------------------------- library */
+const char* miso_last_error(void);
+/* "miniiso <version> gfx950" */
+const char* miso_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

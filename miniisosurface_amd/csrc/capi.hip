@@ -1,0 +1,699 @@
+/*
+ * capi.hip -- host side of libminiiso.so: the C ABI of include/miniiso.h over the HIP kernels.
+ *
+ * Flying Edges keeps the reference's four-pass state machine (flyingEdges/serial/
+ * FlyingEdgesAlgorithm.h:27-49): pass 1 and 2 build the per-row state, pass 3 scans it and sizes
+ * the output (the only host synchronisation, as in the reference where pass 3 allocates the
+ * output vectors, FlyingEdgesAlgorithm.cpp:266-268), pass 4 writes points, normals and triangles.
+ * Device buffers are allocated once per handle and grown only when an output does not fit.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/miniiso.h"
+#include "fe_kernels.h"
+#include "mc_kernels.h"
+
+using namespace miso;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCALL(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? MISO_ERR_ALLOC : MISO_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+template <typename T>
+int dev_alloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(MISO_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", count * sizeof(T),
+                    hipGetErrorString(e));
+    }
+    return MISO_OK;
+}
+
+template <typename T>
+void dev_free(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+int check_volume(const miso_volume* v)
+{
+    if (!v) return fail(MISO_ERR_ARG, "volume is NULL");
+    if (v->nx < 0 || v->ny < 0 || v->nz < 0) return fail(MISO_ERR_ARG, "negative dimension");
+    if (v->nx > 0xFFFFFFFFll) return fail(MISO_ERR_ARG, "nx above 2^32 is not supported");
+    if (!v->data && v->nx * v->ny * v->nz > 0) return fail(MISO_ERR_ARG, "volume data is NULL");
+    return MISO_OK;
+}
+
+Geom make_geom(const miso_volume* v)
+{
+    Geom g{};
+    g.nx = v->nx;
+    g.ny = v->ny;
+    g.nz = v->nz;
+    g.W = (v->nx + 63) / 64;
+    for (int a = 0; a < 3; ++a) {
+        g.org[a] = v->origin[a];
+        g.sp[a] = v->spacing[a];
+    }
+    return g;
+}
+
+inline bool has_cells(const Geom& g) { return g.nx >= 2 && g.ny >= 2 && g.nz >= 2; }
+
+/* Shared per-extraction device state: volume, sign masks, row counts, offsets, scan state. */
+struct RowState {
+    Geom g{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    const float* d_vol = nullptr;
+    float* d_vol_own = nullptr;
+    uint64_t* d_mask = nullptr;
+    RowCounts* d_counts = nullptr;
+    RowTrim* d_trims = nullptr;
+    uint64_t* d_voff = nullptr;
+    uint64_t* d_toff = nullptr;
+    uint64_t* d_scan = nullptr;
+    int64_t ntiles = 0;
+    uint64_t* d_totals = nullptr;
+    uint64_t* h_totals = nullptr;   // pinned
+
+    int init(const miso_volume* v, void* user_stream)
+    {
+        g = make_geom(v);
+        if (user_stream) {
+            stream = (hipStream_t)user_stream;
+        } else {
+            HIPCALL(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            own_stream = true;
+        }
+        const int64_t N = g.nx * g.ny * g.nz;
+        if (v->on_device || N == 0) {
+            d_vol = v->data;
+        } else {
+            if (int rc = dev_alloc(&d_vol_own, (size_t)N)) return rc;
+            HIPCALL(hipMemcpyAsync(d_vol_own, v->data, (size_t)N * sizeof(float), hipMemcpyHostToDevice, stream));
+            HIPCALL(hipStreamSynchronize(stream));   // the host volume is only borrowed until here
+            d_vol = d_vol_own;
+        }
+        HIPCALL(hipHostMalloc((void**)&h_totals, 2 * sizeof(uint64_t), hipHostMallocDefault));
+        h_totals[0] = h_totals[1] = 0;
+        if (!has_cells(g)) return MISO_OK;
+        const int64_t R = g.ny * g.nz;
+        ntiles = (R + kScanTile - 1) / kScanTile;
+        int rc = 0;
+        if ((rc = dev_alloc(&d_mask, (size_t)(R * g.W)))) return rc;
+        if ((rc = dev_alloc(&d_counts, (size_t)R))) return rc;
+        if ((rc = dev_alloc(&d_trims, (size_t)R))) return rc;
+        if ((rc = dev_alloc(&d_voff, (size_t)R))) return rc;
+        if ((rc = dev_alloc(&d_toff, (size_t)R))) return rc;
+        if ((rc = dev_alloc(&d_scan, (size_t)(1 + 2 * ntiles)))) return rc;
+        if ((rc = dev_alloc(&d_totals, 2))) return rc;
+        return MISO_OK;
+    }
+
+    void release()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        dev_free(d_vol_own);
+        dev_free(d_mask);
+        dev_free(d_counts);
+        dev_free(d_trims);
+        dev_free(d_voff);
+        dev_free(d_toff);
+        dev_free(d_scan);
+        dev_free(d_totals);
+        if (h_totals) (void)hipHostFree(h_totals);
+        h_totals = nullptr;
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    unsigned row_blocks() const { return (unsigned)((g.ny * g.nz + kRowsPerBlock - 1) / kRowsPerBlock); }
+
+    int launch_classify(float iso)
+    {
+        hipLaunchKernelGGL(k_fe_classify, dim3(row_blocks()), dim3(256), 0, stream, d_vol, g, iso, d_mask);
+        HIPCALL(hipGetLastError());
+        return MISO_OK;
+    }
+
+    int launch_count()
+    {
+        hipLaunchKernelGGL(k_fe_count, dim3(row_blocks()), dim3(256), 0, stream, (const uint64_t*)d_mask, g,
+                           d_counts, d_trims, d_scan, (int64_t)(1 + 2 * ntiles));
+        HIPCALL(hipGetLastError());
+        return MISO_OK;
+    }
+
+    int launch_scan()
+    {
+        hipLaunchKernelGGL(k_fe_scan, dim3((unsigned)ntiles), dim3(256), 0, stream, (const RowCounts*)d_counts,
+                           g.ny * g.nz, d_voff, d_toff, d_scan, ntiles, d_totals);
+        HIPCALL(hipGetLastError());
+        HIPCALL(hipMemcpyAsync(h_totals, d_totals, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+        return MISO_OK;
+    }
+};
+
+/* Marching Cubes state on top of the Flying Edges row state. */
+struct McState {
+    uint32_t* d_owned = nullptr;     // owned cut edges per cell row (grid-row indexed)
+    uint64_t* d_moff = nullptr;      // Marching Cubes vertex offset per cell row
+    uint64_t* d_scan = nullptr;
+    uint64_t* d_total = nullptr;
+    int64_t ntiles = 0;
+    void* d_map = nullptr;           // Flying Edges vertex id -> Marching Cubes vertex id
+    uint64_t map_bytes = 0;
+
+    int init(const Geom& g)
+    {
+        const int64_t R = g.ny * g.nz;
+        ntiles = (R + kScanTile - 1) / kScanTile;
+        int rc = 0;
+        if ((rc = dev_alloc(&d_owned, (size_t)R))) return rc;
+        if ((rc = dev_alloc(&d_moff, (size_t)R))) return rc;
+        if ((rc = dev_alloc(&d_scan, (size_t)(1 + ntiles)))) return rc;
+        if ((rc = dev_alloc(&d_total, 1))) return rc;
+        return MISO_OK;
+    }
+
+    void release()
+    {
+        dev_free(d_owned);
+        dev_free(d_moff);
+        dev_free(d_scan);
+        dev_free(d_total);
+        if (d_map) (void)hipFree(d_map);
+        d_map = nullptr;
+    }
+
+    int launch_count(RowState& s)
+    {
+        hipLaunchKernelGGL(k_mc_count, dim3(s.row_blocks()), dim3(256), 0, s.stream, (const uint64_t*)s.d_mask,
+                           s.g, (const RowCounts*)s.d_counts, d_owned, d_scan, (int64_t)(1 + ntiles));
+        HIPCALL(hipGetLastError());
+        hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)ntiles), dim3(256), 0, s.stream, (const uint32_t*)d_owned,
+                           s.g.ny * s.g.nz, d_moff, d_scan, ntiles, d_total);
+        HIPCALL(hipGetLastError());
+        return MISO_OK;
+    }
+
+    template <typename Index>
+    int emit(RowState& s, float iso, float* pts, float* nrm, Index* tris, uint64_t nverts)
+    {
+        const uint64_t need = (nverts ? nverts : 1) * sizeof(Index);
+        if (need > map_bytes) {
+            if (d_map) (void)hipFree(d_map);
+            d_map = nullptr;
+            uint8_t* p = nullptr;
+            if (int rc = dev_alloc(&p, (size_t)need)) return rc;
+            d_map = p;
+            map_bytes = need;
+        }
+        hipLaunchKernelGGL(k_mc_vertices<Index>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                           (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
+                           (const uint64_t*)s.d_voff, (const uint64_t*)d_moff, pts, nrm, (Index*)d_map);
+        HIPCALL(hipGetLastError());
+        hipLaunchKernelGGL(k_mc_triangles<Index>, dim3(s.row_blocks()), dim3(256), 0, s.stream,
+                           (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff,
+                           (const uint64_t*)s.d_toff, (const Index*)d_map, tris);
+        HIPCALL(hipGetLastError());
+        return MISO_OK;
+    }
+
+    int launch_emit(RowState& s, float iso, float* pts, float* nrm, void* tris, int index_bits, uint64_t nverts)
+    {
+        if (index_bits == 64) return emit<uint64_t>(s, iso, pts, nrm, (uint64_t*)tris, nverts);
+        return emit<uint32_t>(s, iso, pts, nrm, (uint32_t*)tris, nverts);
+    }
+};
+
+/* Output buffers, grown on demand. */
+struct MeshBuffers {
+    float* d_pts = nullptr;
+    float* d_nrm = nullptr;
+    void* d_tris = nullptr;
+    uint64_t cap_v = 0, cap_t = 0;
+    int cap_t_bits = 0;
+    uint64_t nverts = 0, ntris = 0;
+    int index_bits = 32;
+    bool force64 = false;
+
+    int reserve(uint64_t v, uint64_t t)
+    {
+        nverts = v;
+        ntris = t;
+        index_bits = (force64 || v > 0xFFFFFFFFull) ? 64 : 32;
+        if (v > cap_v || !d_pts) {
+            dev_free(d_pts);
+            dev_free(d_nrm);
+            if (int rc = dev_alloc(&d_pts, (size_t)(3 * v))) return rc;
+            if (int rc = dev_alloc(&d_nrm, (size_t)(3 * v))) return rc;
+            cap_v = v;
+        }
+        const uint64_t need = 3 * t * (uint64_t)(index_bits / 8);
+        if (need > cap_t * (uint64_t)(cap_t_bits / 8) * 3 || !d_tris) {
+            if (d_tris) (void)hipFree(d_tris);
+            d_tris = nullptr;
+            uint8_t* p = nullptr;
+            if (int rc = dev_alloc(&p, (size_t)need)) return rc;
+            d_tris = p;
+            cap_t = t;
+            cap_t_bits = index_bits;
+        }
+        return MISO_OK;
+    }
+
+    void release()
+    {
+        dev_free(d_pts);
+        dev_free(d_nrm);
+        if (d_tris) (void)hipFree(d_tris);
+        d_tris = nullptr;
+    }
+
+    int copy_out(hipStream_t s, float* pts, float* nrm, uint64_t* tris)
+    {
+        if (pts && nverts) HIPCALL(hipMemcpyAsync(pts, d_pts, 3 * nverts * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (nrm && nverts) HIPCALL(hipMemcpyAsync(nrm, d_nrm, 3 * nverts * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (tris && ntris) {
+            if (index_bits == 64) {
+                HIPCALL(hipMemcpyAsync(tris, d_tris, 3 * ntris * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+                HIPCALL(hipStreamSynchronize(s));
+            } else {
+                // widen 32-bit indices to the reference's size_t in place: copy into the upper
+                // half of the caller's buffer, then expand from the front (no overlap hazard)
+                uint32_t* tmp = reinterpret_cast<uint32_t*>(tris) + 3 * ntris;
+                HIPCALL(hipMemcpyAsync(tmp, d_tris, 3 * ntris * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                HIPCALL(hipStreamSynchronize(s));
+                for (uint64_t q = 0; q < 3 * ntris; ++q) tris[q] = tmp[q];
+            }
+        }
+        HIPCALL(hipStreamSynchronize(s));
+        return MISO_OK;
+    }
+};
+
+/* true case of each cell, or the reference's trimmed convention (0 outside [xl, xr)) */
+__global__ void k_fe_cube_cases(const uint64_t* __restrict__ mask, Geom g, const RowTrim* __restrict__ trims,
+                                uint8_t* __restrict__ out, int all_cells)
+{
+    const int64_t cx = g.nx - 1, cy = g.ny - 1, cz = g.nz - 1;
+    const int64_t n = cx * cy * cz;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = c % cx, j = (c / cx) % cy, k = c / (cx * cy);
+        const int64_t r = j + g.ny * k;
+        const int64_t rows[4] = {r, r + 1, r + g.ny, r + g.ny + 1};
+        if (!all_cells) {
+            // calcTrimValues (FlyingEdgesAlgorithm.cpp:606-621)
+            uint32_t xl = trims[rows[0]].xl, xr = trims[rows[0]].xr;
+            for (int q = 1; q < 4; ++q) {
+                xl = min(xl, trims[rows[q]].xl);
+                xr = max(xr, trims[rows[q]].xr);
+            }
+            if (xl > xr) xl = xr;
+            if (i < (int64_t)xl || i >= (int64_t)xr) {
+                out[c] = 0;
+                continue;
+            }
+        }
+        auto bit = [&](int q, int64_t x) -> int {
+            return (int)((mask[rows[q] * g.W + (x >> 6)] >> (x & 63)) & 1ull);
+        };
+        out[c] = (uint8_t)(bit(0, i) | bit(0, i + 1) << 1 | bit(1, i + 1) << 2 | bit(1, i) << 3 |
+                           bit(2, i) << 4 | bit(2, i + 1) << 5 | bit(3, i + 1) << 6 | bit(3, i) << 7);
+    }
+}
+
+}  // namespace
+
+/* ====================================================================== Flying Edges */
+
+struct miso_fe {
+    RowState st;
+    MeshBuffers out;
+    float iso = 0.0f;
+    int done = 0;                    // last completed pass
+    hipEvent_t ev[8] = {};           // start/stop of passes 1..4
+    bool events = false;
+    float pass_ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fe_pass_launch(miso_fe* h, int pass)
+{
+    RowState& s = h->st;
+    if (!has_cells(s.g)) return MISO_OK;
+    HIPCALL(hipEventRecord(h->ev[2 * (pass - 1)], s.stream));
+    switch (pass) {
+    case 1:
+        if (int rc = s.launch_classify(h->iso)) return rc;
+        break;
+    case 2:
+        if (int rc = s.launch_count()) return rc;
+        break;
+    case 3:
+        if (int rc = s.launch_scan()) return rc;
+        break;
+    case 4: {
+        const MeshBuffers& o = h->out;
+        if (o.nverts || o.ntris) {
+            if (o.index_bits == 64)
+                hipLaunchKernelGGL(k_fe_emit<uint64_t>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                                   (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,
+                                   (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm,
+                                   (uint64_t*)o.d_tris);
+            else
+                hipLaunchKernelGGL(k_fe_emit<uint32_t>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                                   (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,
+                                   (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm,
+                                   (uint32_t*)o.d_tris);
+            HIPCALL(hipGetLastError());
+        }
+        break;
+    }
+    default:
+        return fail(MISO_ERR_ARG, "pass must be 1..4");
+    }
+    HIPCALL(hipEventRecord(h->ev[2 * (pass - 1) + 1], s.stream));
+    return MISO_OK;
+}
+
+int fe_after_scan(miso_fe* h)
+{
+    RowState& s = h->st;
+    HIPCALL(hipStreamSynchronize(s.stream));
+    uint64_t v = 0, t = 0;
+    if (has_cells(s.g)) {
+        v = s.h_totals[0];
+        t = s.h_totals[1];
+    }
+    return h->out.reserve(v, t);
+}
+
+int fe_collect_times(miso_fe* h)
+{
+    if (!has_cells(h->st.g)) return MISO_OK;
+    for (int p = 0; p < 4; ++p) HIPCALL(hipEventElapsedTime(&h->pass_ms[p], h->ev[2 * p], h->ev[2 * p + 1]));
+    return MISO_OK;
+}
+
+}  // namespace
+
+extern "C" int miso_fe_create(const miso_volume* vol, float isoval, void* stream, miso_fe** out)
+{
+    if (!out) return fail(MISO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (int rc = check_volume(vol)) return rc;
+    miso_fe* h = new (std::nothrow) miso_fe();
+    if (!h) return fail(MISO_ERR_ALLOC, "host allocation failed");
+    h->iso = isoval;
+    int rc = h->st.init(vol, stream);
+    if (rc == MISO_OK) {
+        for (int e = 0; e < 8 && rc == MISO_OK; ++e)
+            if (hipEventCreate(&h->ev[e]) != hipSuccess) rc = fail(MISO_ERR_HIP, "hipEventCreate failed");
+        h->events = rc == MISO_OK;
+    }
+    if (rc != MISO_OK) {
+        miso_fe_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_set_isoval(miso_fe* h, float isoval)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    h->iso = isoval;
+    h->done = 0;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_pass(miso_fe* h, int pass)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (pass < 1 || pass > 4) return fail(MISO_ERR_ARG, "pass must be 1..4");
+    if (pass != h->done + 1 && !(pass == 1)) return fail(MISO_ERR_STATE, "pass %d called after pass %d", pass, h->done);
+    if (int rc = fe_pass_launch(h, pass)) return rc;
+    if (pass == 3) {
+        if (int rc = fe_after_scan(h)) return rc;
+    } else {
+        HIPCALL(hipStreamSynchronize(h->st.stream));
+    }
+    if (pass == 4) fe_collect_times(h);
+    h->done = pass;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_run(miso_fe* h)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    for (int p = 1; p <= 3; ++p)
+        if (int rc = fe_pass_launch(h, p)) return rc;
+    if (int rc = fe_after_scan(h)) return rc;
+    if (int rc = fe_pass_launch(h, 4)) return rc;
+    HIPCALL(hipStreamSynchronize(h->st.stream));
+    h->done = 4;
+    return fe_collect_times(h);
+}
+
+extern "C" int miso_fe_counts(const miso_fe* h, uint64_t* nverts, uint64_t* ntris)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (h->done < 3) return fail(MISO_ERR_STATE, "counts are known after pass 3");
+    if (nverts) *nverts = h->out.nverts;
+    if (ntris) *ntris = h->out.ntris;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_copy_out(miso_fe* h, float* points, float* normals, uint64_t* triangles)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (h->done < 4) return fail(MISO_ERR_STATE, "the mesh is complete after pass 4");
+    return h->out.copy_out(h->st.stream, points, normals, triangles);
+}
+
+extern "C" int miso_fe_device_output(miso_fe* h, const float** points, const float** normals,
+                                     const void** triangles, int* index_bits)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (h->done < 4) return fail(MISO_ERR_STATE, "the mesh is complete after pass 4");
+    if (points) *points = h->out.d_pts;
+    if (normals) *normals = h->out.d_nrm;
+    if (triangles) *triangles = h->out.d_tris;
+    if (index_bits) *index_bits = h->out.index_bits;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_set_index_bits(miso_fe* h, int index_bits)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (index_bits != 32 && index_bits != 64) return fail(MISO_ERR_ARG, "index_bits must be 32 or 64");
+    h->out.force64 = index_bits == 64;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_pass_times(const miso_fe* h, float ms[4])
+{
+    if (!h || !ms) return fail(MISO_ERR_ARG, "NULL argument");
+    for (int p = 0; p < 4; ++p) ms[p] = h->pass_ms[p];
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_rows(const miso_fe* h, uint32_t* xl, uint32_t* xr, uint32_t* xcount, uint32_t* ycount,
+                            uint32_t* zcount, uint32_t* tcount, uint64_t* voffset, uint64_t* toffset)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (h->done < 3) return fail(MISO_ERR_STATE, "row state is complete after pass 3");
+    const RowState& s = h->st;
+    if (!has_cells(s.g)) return MISO_OK;
+    const size_t R = (size_t)(s.g.ny * s.g.nz);
+    std::vector<RowCounts> c(R);
+    std::vector<RowTrim> t(R);
+    HIPCALL(hipMemcpy(c.data(), s.d_counts, R * sizeof(RowCounts), hipMemcpyDeviceToHost));
+    HIPCALL(hipMemcpy(t.data(), s.d_trims, R * sizeof(RowTrim), hipMemcpyDeviceToHost));
+    if (voffset) HIPCALL(hipMemcpy(voffset, s.d_voff, R * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (toffset) HIPCALL(hipMemcpy(toffset, s.d_toff, R * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < R; ++r) {
+        if (xl) xl[r] = t[r].xl;
+        if (xr) xr[r] = t[r].xr;
+        if (xcount) xcount[r] = c[r].x;
+        if (ycount) ycount[r] = c[r].y;
+        if (zcount) zcount[r] = c[r].z;
+        if (tcount) tcount[r] = c[r].t;
+    }
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_cube_cases(miso_fe* h, uint8_t* cases, int all_cells)
+{
+    if (!h || !cases) return fail(MISO_ERR_ARG, "NULL argument");
+    if (h->done < 2) return fail(MISO_ERR_STATE, "cube cases are known after pass 2");
+    const RowState& s = h->st;
+    if (!has_cells(s.g)) return MISO_OK;
+    const size_t n = (size_t)((s.g.nx - 1) * (s.g.ny - 1) * (s.g.nz - 1));
+    uint8_t* d = nullptr;
+    if (int rc = dev_alloc(&d, n)) return rc;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fe_cube_cases, dim3(blocks), dim3(256), 0, s.stream, (const uint64_t*)s.d_mask, s.g,
+                       (const RowTrim*)s.d_trims, d, all_cells);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(cases, d, n, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(MISO_ERR_HIP, "cube case dump failed: %s", hipGetErrorString(e));
+    return MISO_OK;
+}
+
+extern "C" void miso_fe_destroy(miso_fe* h)
+{
+    if (!h) return;
+    h->st.release();
+    h->out.release();
+    for (int e = 0; e < 8; ++e)
+        if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
+    delete h;
+}
+
+/* ====================================================================== Marching Cubes */
+
+struct miso_mc {
+    RowState st;
+    MeshBuffers out;
+    float iso = 0.0f;
+    bool done = false;
+    McState mc;
+    hipEvent_t ev[2] = {};
+    float ms = 0.0f;
+};
+
+extern "C" int miso_mc_create(const miso_volume* vol, float isoval, void* stream, miso_mc** out)
+{
+    if (!out) return fail(MISO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (int rc = check_volume(vol)) return rc;
+    miso_mc* h = new (std::nothrow) miso_mc();
+    if (!h) return fail(MISO_ERR_ALLOC, "host allocation failed");
+    h->iso = isoval;
+    int rc = h->st.init(vol, stream);
+    if (rc == MISO_OK && has_cells(h->st.g)) rc = h->mc.init(h->st.g);
+    for (int e = 0; e < 2 && rc == MISO_OK; ++e)
+        if (hipEventCreate(&h->ev[e]) != hipSuccess) rc = fail(MISO_ERR_HIP, "hipEventCreate failed");
+    if (rc != MISO_OK) {
+        miso_mc_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return MISO_OK;
+}
+
+extern "C" int miso_mc_run(miso_mc* h)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    RowState& s = h->st;
+    if (!has_cells(s.g)) {
+        h->done = true;
+        return h->out.reserve(0, 0);
+    }
+    HIPCALL(hipEventRecord(h->ev[0], s.stream));
+    if (int rc = s.launch_classify(h->iso)) return rc;
+    if (int rc = s.launch_count()) return rc;
+    if (int rc = s.launch_scan()) return rc;  // FE row offsets: the rank -> id map layout
+    if (int rc = h->mc.launch_count(s)) return rc;
+    HIPCALL(hipStreamSynchronize(s.stream));
+    if (int rc = h->out.reserve(s.h_totals[0], s.h_totals[1])) return rc;
+    if (h->out.nverts || h->out.ntris)
+        if (int rc = h->mc.launch_emit(s, h->iso, h->out.d_pts, h->out.d_nrm, h->out.d_tris, h->out.index_bits,
+                                       h->out.nverts))
+            return rc;
+    HIPCALL(hipEventRecord(h->ev[1], s.stream));
+    HIPCALL(hipStreamSynchronize(s.stream));
+    HIPCALL(hipEventElapsedTime(&h->ms, h->ev[0], h->ev[1]));
+    h->done = true;
+    return MISO_OK;
+}
+
+extern "C" int miso_mc_counts(const miso_mc* h, uint64_t* nverts, uint64_t* ntris)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (!h->done) return fail(MISO_ERR_STATE, "run first");
+    if (nverts) *nverts = h->out.nverts;
+    if (ntris) *ntris = h->out.ntris;
+    return MISO_OK;
+}
+
+extern "C" int miso_mc_copy_out(miso_mc* h, float* points, float* normals, uint64_t* triangles)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (!h->done) return fail(MISO_ERR_STATE, "run first");
+    return h->out.copy_out(h->st.stream, points, normals, triangles);
+}
+
+extern "C" int miso_mc_device_output(miso_mc* h, const float** points, const float** normals,
+                                     const void** triangles, int* index_bits)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (!h->done) return fail(MISO_ERR_STATE, "run first");
+    if (points) *points = h->out.d_pts;
+    if (normals) *normals = h->out.d_nrm;
+    if (triangles) *triangles = h->out.d_tris;
+    if (index_bits) *index_bits = h->out.index_bits;
+    return MISO_OK;
+}
+
+extern "C" int miso_mc_time_ms(const miso_mc* h, float* ms)
+{
+    if (!h || !ms) return fail(MISO_ERR_ARG, "NULL argument");
+    *ms = h->ms;
+    return MISO_OK;
+}
+
+extern "C" void miso_mc_destroy(miso_mc* h)
+{
+    if (!h) return;
+    h->st.release();
+    h->mc.release();
+    h->out.release();
+    for (int e = 0; e < 2; ++e)
+        if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
+    delete h;
+}
+
+/* ====================================================================== library */
+
+extern "C" const char* miso_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char* miso_version(void) { return "miniiso 0.1 gfx950"; }

@@ -1,0 +1,197 @@
+"""Closed-form restatement of the isosurface outputs in numpy, used to pin the oracle.
+
+Everything here is derived from the sign field s = (v >= iso) and the FP32 formulas, not from
+the reference's pass structure:
+  * a grid row (j,k) holds its cut x-edges, then its cut y-edges (to row j+1), then its cut
+    z-edges (to plane k+1), each by increasing i; rows follow k-major order;
+  * triangles follow cells in (k, j, i) order, each cell's triangles in case-table order;
+  * Marching Cubes numbers a vertex at its owner cell (the adjacent cell with the smallest
+    (k, j, i)), by the edge's first slot in the owner's triangle list;
+  * coordinates: origin + float(index)*spacing, the far corner adds spacing (Image3D.cpp:45-87);
+    gradients one-sided at faces, central inside, (lower - upper)/run (Image3D.cpp:116-188);
+    interpolation a + w*(b - a) in float32 without fused multiply-add.
+"""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+F = np.float32
+
+# cube corner offsets (i, j, k) of corners 0..7 and edge endpoint corners (lower -> higher)
+CORNER = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+EDGE_LO = [0, 1, 3, 0, 4, 5, 7, 4, 0, 1, 3, 2]
+EDGE_HI = [1, 2, 2, 3, 5, 6, 6, 7, 4, 5, 7, 6]
+
+
+def product_case_table():
+    """Triangle edge lists per case, parsed from the packed table the HIP kernels use."""
+    src = open(os.path.join(ROOT, "miniisosurface_amd", "csrc", "tables.h")).read()
+    body = src[src.index("kCaseEdges[256]"):]
+    body = body[:body.index("};")]
+    words = [int(x, 16) for x in re.findall(r"0x([0-9a-f]{16})ull", body)]
+    assert len(words) == 256
+    table = []
+    for w in words:
+        edges = []
+        for s in range(15):
+            e = (w >> (4 * s)) & 0xF
+            if e == 0xF:
+                break
+            edges.append(e)
+        table.append(edges)
+    return table
+
+
+def edge_geometry(e):
+    lo, hi = CORNER[EDGE_LO[e]], CORNER[EDGE_HI[e]]
+    axis = [hi[a] - lo[a] for a in range(3)].index(1)
+    return axis, lo  # axis and lower-corner offset (dx, dy, dz)
+
+
+def _coord(o, s, idx, n):
+    """FE coordinate of grid index idx (vectorised), float32."""
+    idx = np.asarray(idx)
+    base = F(o) + idx.astype(F) * F(s)
+    last = (F(o) + F(n - 2) * F(s)) + F(s)
+    return np.where(idx < n - 1, base, last).astype(F)
+
+
+def _grad(vol, i, j, k, spacing):
+    """Gradient (3 components) at grid vertices (i,j,k), float32."""
+    nz, ny, nx = vol.shape
+    out = []
+    for a, (n, s) in enumerate(zip((nx, ny, nz), spacing)):
+        pos = (i, j, k)[a]
+        step = [0, 0, 0]
+        step[a] = 1
+
+        def at(d):
+            ii, jj, kk = i + step[0] * d, j + step[1] * d, k + step[2] * d
+            ii = np.clip(ii, 0, nx - 1)
+            jj = np.clip(jj, 0, ny - 1)
+            kk = np.clip(kk, 0, nz - 1)
+            return vol[kk, jj, ii]
+        first = pos == 0
+        lastp = pos == n - 1
+        hi = np.where(first, at(1), np.where(lastp, at(0), at(1)))
+        lo = np.where(first, at(0), np.where(lastp, at(-1), at(-1)))
+        run = np.where(first | lastp, F(s), F(2) * F(s)).astype(F)
+        with np.errstate(all="ignore"):
+            out.append(((lo - hi) / run).astype(F))
+    return out
+
+
+def _interp(iso, v0, v1, a, b):
+    with np.errstate(all="ignore"):
+        w = ((F(iso) - v0) / (v1 - v0)).astype(F)
+        return (a + (w * (b - a)).astype(F)).astype(F)
+
+
+def fe_closed_form(vol, iso, spacing=(1, 1, 1), origin=(0, 0, 0), table=None):
+    """(points, normals, triangles, rowcounts) of Flying Edges from the closed form."""
+    vol = np.ascontiguousarray(vol, F)
+    nz, ny, nx = vol.shape
+    table = table or product_case_table()
+    if min(nx, ny, nz) < 2:
+        return np.zeros((0, 3), F), np.zeros((0, 3), F), np.zeros((0, 3), np.uint64), None
+    s = vol >= F(iso)
+    cut = [np.zeros((nz, ny, nx), bool) for _ in range(3)]
+    cut[0][:, :, :-1] = s[:, :, :-1] != s[:, :, 1:]
+    cut[1][:, :-1, :] = s[:, :-1, :] != s[:, 1:, :]
+    cut[2][:-1, :, :] = s[:-1, :, :] != s[1:, :, :]
+    cnt = np.stack([c.sum(axis=2) for c in cut], axis=-1)       # (nz, ny, 3)
+    per_row = cnt.sum(axis=-1).reshape(-1)
+    row_off = np.concatenate([[0], np.cumsum(per_row)])[:-1].reshape(nz, ny)
+    ids = []
+    for a in range(3):
+        rank = np.cumsum(cut[a], axis=2) - 1
+        block = row_off + (cnt[:, :, :a].sum(axis=-1) if a else 0)
+        ids.append(np.where(cut[a], block[:, :, None] + rank, -1).astype(np.int64))
+    V = int(per_row.sum())
+    pts = np.zeros((V, 3), F)
+    nrm = np.zeros((V, 3), F)
+    for a in range(3):
+        k, j, i = np.nonzero(cut[a])
+        vid = ids[a][k, j, i]
+        step = [0, 0, 0]
+        step[a] = 1
+        v0 = vol[k, j, i]
+        v1 = vol[k + step[2], j + step[1], i + step[0]]
+        A = [_coord(origin[0], spacing[0], i, nx), _coord(origin[1], spacing[1], j, ny),
+             _coord(origin[2], spacing[2], k, nz)]
+        B = list(A)
+        B[a] = (A[a] + F(spacing[a])).astype(F)
+        gA = _grad(vol, i, j, k, spacing)
+        gB = _grad(vol, i + step[0], j + step[1], k + step[2], spacing)
+        for c in range(3):
+            pts[vid, c] = _interp(iso, v0, v1, A[c], B[c])
+            nrm[vid, c] = _interp(iso, v0, v1, gA[c], gB[c])
+    # triangles: cells in (k, j, i) order
+    corner_bits = np.zeros((nz - 1, ny - 1, nx - 1), np.int64)
+    for q, (di, dj, dk) in enumerate(CORNER):
+        corner_bits |= s[dk:nz - 1 + dk, dj:ny - 1 + dj, di:nx - 1 + di].astype(np.int64) << q
+    ntri = np.array([len(t) // 3 for t in table])
+    tris = []
+    for k, j, i in zip(*np.nonzero(ntri[corner_bits] > 0)):
+        c = corner_bits[k, j, i]
+        for t in range(0, len(table[c]), 3):
+            tri = []
+            for e in table[c][t:t + 3]:
+                axis, (dx, dy, dz) = edge_geometry(e)
+                tri.append(ids[axis][k + dz, j + dy, i + dx])
+            tris.append(tri)
+    tris = np.array(tris, np.uint64).reshape(-1, 3)
+    return pts, nrm, tris, cnt
+
+
+def mc_closed_form(vol, iso, spacing=(1, 1, 1), origin=(0, 0, 0), table=None):
+    """(points, normals, triangles) of serial Marching Cubes: first-visit numbering over cells in
+    (k, j, i) order with a Python dict, written independently of the oracle's C hash map."""
+    vol = np.ascontiguousarray(vol, F)
+    nz, ny, nx = vol.shape
+    table = table or product_case_table()
+    if min(nx, ny, nz) < 2:
+        return np.zeros((0, 3), F), np.zeros((0, 3), F), np.zeros((0, 3), np.uint64)
+    s = vol >= F(iso)
+    ids = {}
+    pts, nrm, tris = [], [], []
+    X = lambda i: F(origin[0]) + F(i) * F(spacing[0])  # noqa: E731
+    Y = lambda j: F(origin[1]) + F(j) * F(spacing[1])  # noqa: E731
+    Z = lambda k: F(origin[2]) + F(k) * F(spacing[2])  # noqa: E731
+    for k in range(nz - 1):
+        for j in range(ny - 1):
+            for i in range(nx - 1):
+                c = 0
+                for q, (di, dj, dk) in enumerate(CORNER):
+                    c |= int(s[k + dk, j + dj, i + di]) << q
+                if not table[c]:
+                    continue
+                for t in range(0, len(table[c]), 3):
+                    tri = []
+                    for e in table[c][t:t + 3]:
+                        axis, (dx, dy, dz) = edge_geometry(e)
+                        key = (axis, i + dx, j + dy, k + dz)
+                        if key not in ids:
+                            ids[key] = len(pts)
+                            A = [X(i) + F(spacing[0]) if dx else X(i), Y(j) + F(spacing[1]) if dy else Y(j),
+                                 Z(k) + F(spacing[2]) if dz else Z(k)]
+                            A = [F(x) for x in A]
+                            B = list(A)
+                            B[axis] = F(A[axis] + F(spacing[axis]))
+                            gi, gj, gk = i + dx, j + dy, k + dz
+                            st = [0, 0, 0]
+                            st[axis] = 1
+                            v0 = vol[gk, gj, gi]
+                            v1 = vol[gk + st[2], gj + st[1], gi + st[0]]
+                            gA = _grad(vol, np.array([gi]), np.array([gj]), np.array([gk]), spacing)
+                            gB = _grad(vol, np.array([gi + st[0]]), np.array([gj + st[1]]), np.array([gk + st[2]]), spacing)
+                            pts.append([_interp(iso, v0, v1, A[q], B[q]) for q in range(3)])
+                            nrm.append([_interp(iso, v0, v1, gA[q][0], gB[q][0]) for q in range(3)])
+                        tri.append(ids[key])
+                    tris.append(tri)
+    return (np.array(pts, F).reshape(-1, 3), np.array(nrm, F).reshape(-1, 3),
+            np.array(tris, np.uint64).reshape(-1, 3))
