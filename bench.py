@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--iso", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -120,12 +121,16 @@ def main():
     kern_total = float(pass_ms.sum())
 
     # the sparse isovalue of the same config
-    algo.set_isoval(0.95)
-    run_steps(1)
-    t2 = time.perf_counter()
-    sp_ms = run_steps(5)
-    sparse_wall = (time.perf_counter() - t2) / 5
-    sV, sT = algo.counts()
+    sparse = None
+    if not args.no_sparse:
+        algo.set_isoval(0.95)
+        run_steps(1)
+        t2 = time.perf_counter()
+        sp_ms = run_steps(5)
+        sparse_wall = (time.perf_counter() - t2) / 5
+        sV, sT = algo.counts()
+        sparse = {"ms_per_step": round(sparse_wall * 1e3, 4), "nverts": sV, "ntris": sT,
+                  "kernel_ms": round(float(sp_ms.sum()), 4), "gcells_s": round(cells / sparse_wall / 1e9, 3)}
 
     out = {
         "metric": "FE Gcells/s & %HBM peak at 1024^3",
@@ -148,9 +153,7 @@ def main():
                        "hbm_frac_kernels": round(b_alg / (kern_total * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4),
                        "pass_ms": [round(float(x), 4) for x in pass_ms]},
-        "sparse_iso_0.95": {"ms_per_step": round(sparse_wall * 1e3, 4), "nverts": sV, "ntris": sT,
-                            "kernel_ms": round(float(sp_ms.sum()), 4),
-                            "gcells_s": round(cells / sparse_wall / 1e9, 3)},
+        "sparse_iso_0.95": sparse,
         "volume_gen_s": round(gen_s, 2),
     }
     if not args.no_cpu:

@@ -16,9 +16,10 @@
 #include <string>
 #include <vector>
 
-#include "../../include/miniiso.h"
+#include "miniiso.h"
 #include "fe_kernels.h"
 #include "mc_kernels.h"
+#include "fe_rows.h"
 
 using namespace miso;
 
@@ -142,6 +143,7 @@ struct RowState {
         return MISO_OK;
     }
 
+
     void release()
     {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -161,6 +163,28 @@ struct RowState {
 
     unsigned row_blocks() const { return (unsigned)((g.ny * g.nz + kRowsPerBlock - 1) / kRowsPerBlock); }
 
+    /* unit spacing, zero origin: coordinates are float(index) and the gradient divisions by 1
+     * and 2 are exact (index < 2^24 keeps float(index) exact) */
+    bool unit_geometry() const
+    {
+        return g.sp[0] == 1.0f && g.sp[1] == 1.0f && g.sp[2] == 1.0f && g.org[0] == 0.0f && g.org[1] == 0.0f &&
+               g.org[2] == 0.0f && g.nx < (1 << 24) && g.ny < (1 << 24) && g.nz < (1 << 24);
+    }
+
+    /* (row, word)-lane kernels: rows of up to 64 mask words (nx <= 4096) */
+    bool lane_rows() const { return g.W <= 64; }
+    int rows_per_wave() const
+    {
+        int wp = 1;
+        while (wp < g.W) wp <<= 1;
+        return 64 / wp;
+    }
+    unsigned lane_row_blocks() const
+    {
+        const int64_t per_block = 4 * (int64_t)rows_per_wave();
+        return (unsigned)((g.ny * g.nz + per_block - 1) / per_block);
+    }
+
     int launch_classify(float iso)
     {
         hipLaunchKernelGGL(k_fe_classify, dim3(row_blocks()), dim3(256), 0, stream, d_vol, g, iso, d_mask);
@@ -170,8 +194,12 @@ struct RowState {
 
     int launch_count()
     {
-        hipLaunchKernelGGL(k_fe_count, dim3(row_blocks()), dim3(256), 0, stream, (const uint64_t*)d_mask, g,
-                           d_counts, d_trims, d_scan, (int64_t)(1 + 2 * ntiles));
+        if (lane_rows())
+            hipLaunchKernelGGL(k_fe_count_rows, dim3(lane_row_blocks()), dim3(256), 0, stream,
+                               (const uint64_t*)d_mask, g, d_counts, d_trims, d_scan, (int64_t)(1 + 2 * ntiles));
+        else
+            hipLaunchKernelGGL(k_fe_count, dim3(row_blocks()), dim3(256), 0, stream, (const uint64_t*)d_mask, g,
+                               d_counts, d_trims, d_scan, (int64_t)(1 + 2 * ntiles));
         HIPCALL(hipGetLastError());
         return MISO_OK;
     }
@@ -389,7 +417,29 @@ int fe_pass_launch(miso_fe* h, int pass)
         break;
     case 4: {
         const MeshBuffers& o = h->out;
-        if (o.nverts || o.ntris) {
+        if ((o.nverts || o.ntris) && s.lane_rows()) {
+            if (o.nverts) {
+                if (s.unit_geometry())
+                    hipLaunchKernelGGL(k_fe_vertices<true>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                                       (const uint64_t*)s.d_mask, s.g, h->iso, (const uint64_t*)s.d_voff, o.d_pts,
+                                       o.d_nrm);
+                else
+                    hipLaunchKernelGGL(k_fe_vertices<false>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                                       (const uint64_t*)s.d_mask, s.g, h->iso, (const uint64_t*)s.d_voff, o.d_pts,
+                                       o.d_nrm);
+            }
+            if (o.ntris) {
+                if (o.index_bits == 64)
+                    hipLaunchKernelGGL(k_fe_triangles<uint64_t>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                                       (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,
+                                       (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (uint64_t*)o.d_tris);
+                else
+                    hipLaunchKernelGGL(k_fe_triangles<uint32_t>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                                       (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,
+                                       (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (uint32_t*)o.d_tris);
+            }
+            HIPCALL(hipGetLastError());
+        } else if (o.nverts || o.ntris) {
             if (o.index_bits == 64)
                 hipLaunchKernelGGL(k_fe_emit<uint64_t>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
                                    (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,

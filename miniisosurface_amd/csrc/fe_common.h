@@ -60,11 +60,12 @@ __device__ __forceinline__ uint64_t low_bits(int64_t count)
     return count >= 64 ? ~0ull : (count <= 0 ? 0ull : ((1ull << count) - 1ull));
 }
 
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v);
+
+/* sum over the 64 lanes, in every lane */
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan_u32(v), 63);
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
@@ -74,15 +75,33 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
     return v;
 }
 
+/* DPP lane moves (VALU data-parallel primitives, no LDS round trip). row_shr:o moves lane i-o
+ * to lane i within each row of 16 lanes (0 where i-o leaves the row); row_bcast:15 / :31 copy
+ * lane 15 / 31 of the preceding rows into the rows selected by the row mask. */
+template <int Ctrl, int RowMask = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, Ctrl, RowMask, 0xF, true);
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+
+/* inclusive prefix sum within each row of 16 lanes */
+__device__ __forceinline__ uint32_t row16_incl_scan_u32(uint32_t v)
+{
+    v += dpp_u32<kDppRowShr1>(v);
+    v += dpp_u32<kDppRowShr2>(v);
+    v += dpp_u32<kDppRowShr4>(v);
+    v += dpp_u32<kDppRowShr8>(v);
+    return v;
+}
+
 /* inclusive prefix sum across the 64 lanes */
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v)
 {
-    const int lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t n = __shfl_up(v, (unsigned)o);
-        if (lane >= o) v += n;
-    }
+    v = row16_incl_scan_u32(v);
+    v += dpp_u32<kDppRowBcast15, 0xA>(v);
+    v += dpp_u32<kDppRowBcast31, 0xC>(v);
     return v;
 }
 

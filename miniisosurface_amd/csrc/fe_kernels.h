@@ -406,7 +406,7 @@ struct RowWalker {
     __device__ uint64_t fe_vertex_id(const uint64_t cbase[8], int e, int li) const
     {
         const int q = combo_of_edge(e);
-        const int p = li + kEdgeDx[e];
+        const int p = li + edge_dx(e);
         const int w = p >> 6, b = p & 63;
         const uint64_t below = L.cut[q][w] & ((1ull << b) - 1ull);
         return cbase[q] + L.pre[q][w] + (uint64_t)__popcll(below);

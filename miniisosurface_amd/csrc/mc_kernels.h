@@ -27,11 +27,8 @@ namespace miso {
 __device__ __forceinline__ uint32_t case_cut_mask(uint64_t packed)
 {
     uint32_t m = 0;
-    for (int s = 0; s < 15; ++s) {
-        const int e = case_edge(packed, s);
-        if (e == 0xF) break;
-        m |= 1u << e;
-    }
+    const int ne = 3 * case_tri_count(packed);
+    for (int s = 0; s < ne; ++s) m |= 1u << case_edge(packed, s);
     return m;
 }
 
@@ -190,13 +187,13 @@ __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ v
                 const uint64_t packed = kCaseEdges[cs];
                 uint64_t vid = mrun + incl - o;
                 uint32_t seen = 0;
-                for (int s = 0; s < 15; ++s) {
+                const int ne = 3 * case_tri_count(packed);
+                for (int s = 0; s < ne; ++s) {
                     const int ed = case_edge(packed, s);
-                    if (ed == 0xF) break;
                     if (!((own >> ed) & 1u) || ((seen >> ed) & 1u)) continue;
                     seen |= 1u << ed;
-                    const int a = kEdgeAxis[ed];
-                    const int dx = kEdgeDx[ed], dy = kEdgeDy[ed], dz = kEdgeDz[ed];
+                    const int a = edge_axis(ed);
+                    const int dx = edge_dx(ed), dy = edge_dy(ed), dz = edge_dz(ed);
                     // corner coordinates as the owner cell computes them (getPosCube)
                     float A[3] = {dx ? X + g.sp[0] : X, dy ? Y + g.sp[1] : Y, dz ? Z + g.sp[2] : Z};
                     float B[3] = {A[0], A[1], A[2]};

@@ -42,6 +42,7 @@ def product_case_table():
             if e == 0xF:
                 break
             edges.append(e)
+        assert (w >> 60) == len(edges) // 3, "top nibble must hold the triangle count"
         table.append(edges)
     return table
 
