@@ -172,7 +172,9 @@ struct RowState {
     }
 
     /* (row, word)-lane kernels: rows of up to 64 mask words (nx <= 4096) */
-    bool lane_rows() const { return g.W <= 64; }
+    /* (row, word) lane kernels: rows of up to 4096 voxels, and the vertex ids a wavefront's
+     * triangles reference (rows r0 .. r0 + 64 + ny + 1) within 2^32 of its first one */
+    bool lane_rows() const { return g.W <= 64 && (g.ny + 130) * 3 * g.nx < (int64_t(1) << 32); }
     int rows_per_wave() const
     {
         int wp = 1;

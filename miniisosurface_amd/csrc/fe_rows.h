@@ -356,6 +356,14 @@ __global__ __launch_bounds__(256) void k_fe_vertices(const float* __restrict__ v
 }
 
 /* ------------------------------------------------------------------ pass 4: triangles */
+constexpr int kTriMapCap = 512;     // triangle-map entries per wavefront and round (2 KB of LDS)
+
+/* The wavefront's triangles are one contiguous range of the output (its rows are consecutive).
+ * Each lane lists the triangles of its word's active cells into an LDS map, in output order
+ * (cells in increasing i, then case-table order); the wavefront then resolves the map one
+ * triangle per lane, so the index stores of neighbouring lanes are contiguous. Vertex ids come
+ * from the owning lane's 8 edge-combination cut words and their id bases, kept in LDS relative
+ * to the wavefront's first vertex (the launch checks that these offsets fit 32 bits). */
 template <typename Index>
 __global__ __launch_bounds__(256) void k_fe_triangles(const uint64_t* __restrict__ mask, Geom g,
                                                       const RowCounts* __restrict__ counts,
@@ -363,20 +371,22 @@ __global__ __launch_bounds__(256) void k_fe_triangles(const uint64_t* __restrict
                                                       const uint64_t* __restrict__ toff,
                                                       Index* __restrict__ tris)
 {
-    // per lane: the 8 combination cut words and their id bases (row offset + block + prefix)
     __shared__ uint64_t s_cut[8][256];
-    __shared__ uint64_t s_base[8][256];
+    __shared__ uint32_t s_base[8][256];
     __shared__ uint64_t s_case[256];
+    __shared__ uint32_t s_map[4][kTriMapCap];
     const int t = threadIdx.x;
+    const int lane = lane_id();
+    const int wv = t >> 6;
     s_case[t] = kCaseEdges[t];
     __syncthreads();
     const RowLanes rl = row_lanes(g.W);
     const int64_t R = g.ny * g.nz;
-    const int64_t wave_id = (int64_t)blockIdx.x * 4 + (t >> 6);
-    const int64_t r = wave_id * rl.rpw + rl.q;
-    const bool in = r < R;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * rl.rpw;
+    if (r0 >= R) return;
+    const int64_t r = r0 + rl.q;
     const int64_t j = r % g.ny, k = r / g.ny;
-    const bool cells = in && j < g.ny - 1 && k < g.nz - 1 && rl.w < g.W;
+    const bool cells = r < R && j < g.ny - 1 && k < g.nz - 1 && rl.w < g.W;
     const uint32_t trow = cells ? counts[r].t : 0u;
     if (__ballot(trow != 0) == 0) return;
     const bool live = cells && trow != 0;
@@ -389,7 +399,9 @@ __global__ __launch_bounds__(256) void k_fe_triangles(const uint64_t* __restrict
     cw[5] = (L.m[2] ^ L.m[3]) & vb;
     cw[6] = (L.m[0] ^ L.m[2]) & vb;
     cw[7] = (L.m[1] ^ L.m[3]) & vb;
-    uint64_t cb[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint64_t v0 = voff[r0];
+    const uint64_t t0 = toff[r0];
+    uint64_t cb[8] = {v0, v0, v0, v0, v0, v0, v0, v0};
     if (live) {
         const int64_t rows[4] = {r, r + 1, r + g.ny, r + g.ny + 1};
         combo_bases(counts, voff, rows, cb);
@@ -397,36 +409,54 @@ __global__ __launch_bounds__(256) void k_fe_triangles(const uint64_t* __restrict
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         s_cut[q][t] = cw[q];
-        s_base[q][t] = cb[q] + seg_excl_scan((uint32_t)__popcll(cw[q]), rl.Wp, rl.w);
+        s_base[q][t] = (uint32_t)(cb[q] - v0) + seg_excl_scan((uint32_t)__popcll(cw[q]), rl.Wp, rl.w);
     }
-    // triangles of this lane's word, then their slots
-    uint64_t act = live ? active_cells(L, eb) : 0ull;
+    const CornerWords C = corner_words(L);
+    const uint64_t act = live ? active_cells(C, eb) : 0ull;
     uint32_t nt = 0;
     for (uint64_t a = act; a;) {
         const int b = __builtin_ctzll(a);
         a &= a - 1;
-        nt += (uint32_t)case_tri_count(s_case[case_at(L, b)]);
+        nt += (uint32_t)case_tri_count(s_case[case_of(C, b)]);
     }
-    uint64_t slot = (live ? toff[r] : 0ull) + seg_excl_scan(nt, rl.Wp, rl.w);
-    wave_sync();
-    while (act) {
-        const int b = __builtin_ctzll(act);
-        act &= act - 1;
-        const uint64_t packed = s_case[case_at(L, b)];
-        const int ntc = case_tri_count(packed);
-        for (int tt = 0; tt < ntc; ++tt) {
-            uint64_t id[3];
+    const uint32_t incl = wave_incl_scan_u32(nt);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    const uint32_t first = incl - nt;
+    uint32_t* map = s_map[wv];
+    for (uint32_t win = 0; win < total; win += kTriMapCap) {
+        const uint32_t wend = win + kTriMapCap;
+        if (nt && first < wend && first + nt > win) {
+            uint32_t p = first;
+            for (uint64_t a = act; a && p < wend;) {
+                const int b = __builtin_ctzll(a);
+                a &= a - 1;
+                const uint32_t cs = (uint32_t)case_of(C, b);
+                const int n = case_tri_count(s_case[cs]);
+                for (int tt = 0; tt < n; ++tt, ++p)
+                    if (p >= win && p < wend) map[p - win] = (uint32_t)lane | (uint32_t)b << 6 | (uint32_t)tt << 12 | cs << 15;
+            }
+        }
+        wave_sync();
+        const uint32_t n = min((uint32_t)kTriMapCap, total - win);
+        for (uint32_t e = lane; e < n; e += 64) {
+            const uint32_t ent = map[e];
+            const int owner = wv * 64 + (int)(ent & 63);
+            const int b = (int)((ent >> 6) & 63);
+            const int tt = (int)((ent >> 12) & 7);
+            const uint64_t packed = s_case[ent >> 15];
+            uint32_t id[3];
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
-                const int e = case_edge(packed, 3 * tt + s);
-                const int q = combo_of_edge(e);
-                const int p = b + edge_dx(e);
-                const uint64_t below = p == 64 ? s_cut[q][t] : (s_cut[q][t] & ((1ull << p) - 1ull));
-                id[s] = s_base[q][t] + (uint64_t)__popcll(below);
+                const int ed = case_edge(packed, 3 * tt + s);
+                const int q = combo_of_edge(ed);
+                const int p = b + edge_dx(ed);
+                const uint64_t cwq = s_cut[q][owner];
+                const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
+                id[s] = s_base[q][owner] + (uint32_t)__popcll(below);
             }
-            store_tri<Index>(tris, slot, id[0], id[1], id[2]);
-            ++slot;
+            store_tri<Index>(tris, t0 + win + e, v0 + id[0], v0 + id[1], v0 + id[2]);
         }
+        wave_sync();
     }
 }
 
