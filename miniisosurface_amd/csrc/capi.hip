@@ -171,7 +171,6 @@ struct RowState {
                g.org[2] == 0.0f && g.nx < (1 << 24) && g.ny < (1 << 24) && g.nz < (1 << 24);
     }
 
-    /* (row, word)-lane kernels: rows of up to 64 mask words (nx <= 4096) */
     /* (row, word) lane kernels: rows of up to 4096 voxels, and the vertex ids a wavefront's
      * triangles reference (rows r0 .. r0 + 64 + ny + 1) within 2^32 of its first one */
     bool lane_rows() const { return g.W <= 64 && (g.ny + 130) * 3 * g.nx < (int64_t(1) << 32); }
@@ -420,26 +419,19 @@ int fe_pass_launch(miso_fe* h, int pass)
     case 4: {
         const MeshBuffers& o = h->out;
         if ((o.nverts || o.ntris) && s.lane_rows()) {
-            if (o.nverts) {
-                if (s.unit_geometry())
-                    hipLaunchKernelGGL(k_fe_vertices<true>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,
-                                       (const uint64_t*)s.d_mask, s.g, h->iso, (const uint64_t*)s.d_voff, o.d_pts,
-                                       o.d_nrm);
-                else
-                    hipLaunchKernelGGL(k_fe_vertices<false>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,
-                                       (const uint64_t*)s.d_mask, s.g, h->iso, (const uint64_t*)s.d_voff, o.d_pts,
-                                       o.d_nrm);
+#define MISO_EMIT_ROWS(IDX, UNIT)                                                                                  \
+    hipLaunchKernelGGL((k_fe_emit_rows<IDX, UNIT>), dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,    \
+                       (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,                     \
+                       (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm, (IDX*)o.d_tris)
+            const bool unit = s.unit_geometry();
+            if (o.index_bits == 64) {
+                if (unit) MISO_EMIT_ROWS(uint64_t, true);
+                else MISO_EMIT_ROWS(uint64_t, false);
+            } else {
+                if (unit) MISO_EMIT_ROWS(uint32_t, true);
+                else MISO_EMIT_ROWS(uint32_t, false);
             }
-            if (o.ntris) {
-                if (o.index_bits == 64)
-                    hipLaunchKernelGGL(k_fe_triangles<uint64_t>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
-                                       (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,
-                                       (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (uint64_t*)o.d_tris);
-                else
-                    hipLaunchKernelGGL(k_fe_triangles<uint32_t>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
-                                       (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,
-                                       (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (uint32_t*)o.d_tris);
-            }
+#undef MISO_EMIT_ROWS
             HIPCALL(hipGetLastError());
         } else if (o.nverts || o.ntris) {
             if (o.index_bits == 64)

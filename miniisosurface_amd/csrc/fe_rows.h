@@ -8,10 +8,10 @@
  * contiguous range of the output, and its triangles one contiguous range too.
  *
  *   k_fe_count_rows  pass 1b + 2 (trims, x/y/z cut counts, triangle counts per row)
- *   k_fe_vertices    pass 4, vertices: cut edges compacted into an LDS list in output order,
- *                    one vertex per lane, written at consecutive addresses
- *   k_fe_triangles   pass 4, triangles: per active cell, case-table order, vertex ids from row
- *                    offsets plus popcount ranks held in LDS
+ *   k_fe_emit_rows   pass 4: vertices (cut edges compacted into an LDS list in output order, one
+ *                    vertex per lane, written at consecutive addresses), then triangles (active
+ *                    cells listed in output order, resolved one triangle per lane, vertex ids from
+ *                    row offsets plus popcount ranks held in LDS)
  */
 #ifndef MINIISO_FE_ROWS_H
 #define MINIISO_FE_ROWS_H
@@ -19,19 +19,6 @@
 #include "fe_kernels.h"
 
 namespace miso {
-
-constexpr int kListCap = 1024;      // vertex-list entries per wavefront and round (4 KB of LDS)
-constexpr int kXcds = 8;            // MI355X: 8 accelerator complex dies, each with its own L2
-
-/* Blocks are dispatched round-robin over the XCDs (block b runs on XCD b % 8). Renumber them so
- * that each XCD walks one contiguous 1/8 of the rows: the neighbour rows a block reads (j+1 and
- * the next plane) were then brought into the same XCD's L2 by the block just before it. */
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb)
-{
-    const int64_t per = nb / kXcds, extra = nb % kXcds;
-    const int64_t x = b % kXcds, i = b / kXcds;
-    return x * per + min(x, extra) + i;
-}
 
 struct RowLanes {
     int Wp;       // lanes per row (power of two, >= W)
@@ -220,7 +207,7 @@ __global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restric
     }
 }
 
-/* ------------------------------------------------------------------ pass 4: vertices */
+/* ------------------------------------------------------------------ pass 4: edge points */
 
 /* One gradient component, the reference's formula ((lower - upper) / run, one-sided at the
  * faces; flyingEdges/util/Image3D.cpp:116-188). With unit spacing the divisions by 1 and 2 are
@@ -273,190 +260,221 @@ __device__ __forceinline__ void fe_edge_point(const float* __restrict__ d, const
     nrm[2] = ga2 + w * (gb2 - ga2);
 }
 
-template <bool Unit>
-__global__ __launch_bounds__(256) void k_fe_vertices(const float* __restrict__ vol,
-                                                     const uint64_t* __restrict__ mask, Geom g, float iso,
-                                                     const uint64_t* __restrict__ voff,
-                                                     float* __restrict__ pts, float* __restrict__ nrm)
+/* ------------------------------------------------------------------ pass 4: fused emit */
+constexpr int kEmitVertCap = 512;   // vertex-list entries per wavefront and round
+constexpr int kEmitCellCap = 512;   // active-cell entries per wavefront and round
+constexpr int kEmitTriCap = 64 * 5; // triangles of one group of 64 cells (at most 5 each)
+
+/* LDS of one wavefront: the vertex list, then (reusing it) the cell list and triangle map, beside
+ * the 8 edge-combination cut words and id bases of each lane. */
+struct EmitRowsLds {
+    uint64_t cut[8][64];
+    uint32_t base[8][64];
+    union {
+        uint32_t vlist[kEmitVertCap];
+        struct {
+            uint32_t cells[kEmitCellCap];
+            uint32_t map[kEmitTriCap];
+        } t;
+    } u;
+};
+
+/* case of the cell at bit b of a 32-bit half (corner bits v0..v7 as case_of) */
+__device__ __forceinline__ uint32_t case_of_half(const uint32_t h[8], int b)
 {
-    __shared__ uint32_t s_list[4][kListCap];
+    uint32_t c = __builtin_amdgcn_ubfe(h[0], b, 1);
+#pragma unroll
+    for (int q = 1; q < 8; ++q) c |= __builtin_amdgcn_ubfe(h[q], b, 1) << q;
+    return c;
+}
+
+/* Pass 4 for the (row, word) lane layout in one launch: each wavefront writes the vertices of its
+ * rows (one vertex per lane from an LDS list in output order, two per lane per step so that the
+ * gathers of both are in flight together), then the triangles of its cell rows (active cells
+ * listed per lane, resolved one cell and then one triangle per lane). */
+template <typename Index, bool Unit>
+__global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ vol,
+                                                      const uint64_t* __restrict__ mask, Geom g, float iso,
+                                                      const RowCounts* __restrict__ counts,
+                                                      const uint64_t* __restrict__ voff,
+                                                      const uint64_t* __restrict__ toff,
+                                                      float* __restrict__ pts, float* __restrict__ nrm,
+                                                      Index* __restrict__ tris)
+{
+    __shared__ uint64_t s_case[256];
+    __shared__ EmitRowsLds s_w[4];
+    const int t = threadIdx.x;
     const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
-    uint32_t* list = s_list[wv];
+    const int wv = t >> 6;
+    s_case[t] = kCaseEdges[t];
+    __syncthreads();
+    EmitRowsLds& S = s_w[wv];
     const RowLanes rl = row_lanes(g.W);
     const int64_t R = g.ny * g.nz;
-    const int64_t wave_id = (int64_t)blockIdx.x * 4 + wv;
-    const int64_t r0 = wave_id * rl.rpw;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * rl.rpw;
     if (r0 >= R) return;
     const int64_t r = r0 + rl.q;
     const bool ok = r < R && rl.w < g.W;
     const int ny = (int)g.ny;
     const int j = (int)(r % ny), k = (int)(r / ny);
     const bool has_y = j < ny - 1, has_z = k < (int)g.nz - 1;
-    uint64_t m0 = 0, m0n = 0, my = 0, mz = 0;
-    if (ok) {
-        const uint64_t* __restrict__ mr = mask + r * g.W + rl.w;
-        m0 = mr[0];
-        m0n = rl.w + 1 < g.W ? mr[1] : 0ull;
-        my = has_y ? mr[g.W] : 0ull;
-        mz = has_z ? mr[g.ny * g.W] : 0ull;
-    }
-    const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
-    const uint64_t cut[3] = {(m0 ^ shifted(m0, m0n)) & eb, has_y ? ((m0 ^ my) & vb) : 0ull,
-                             has_z ? ((m0 ^ mz) & vb) : 0ull};
-    const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
-    // position of this lane's first x / y / z entry in the wavefront's vertex sequence
-    const uint32_t X = seg_sum(c0, rl.Wp), Y = seg_sum(c1, rl.Wp);
-    const uint32_t lane_tot = c0 + c1 + c2;
-    const uint32_t incl = wave_incl_scan_u32(lane_tot);
-    const int seg_first_lane = rl.q * rl.Wp;
-    const uint32_t before_row = __shfl(incl - lane_tot, seg_first_lane);   // vertices of earlier rows
-    const uint32_t pos0 = before_row + seg_excl_scan(c0, rl.Wp, rl.w);
-    const uint32_t pos1 = before_row + X + seg_excl_scan(c1, rl.Wp, rl.w);
-    const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
-    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-    if (total == 0) return;
-    const uint64_t vbase = voff[r0];
-
-    for (uint32_t win = 0; win < total; win += kListCap) {
-        // write the entries whose position falls in [win, win + kListCap): row q, axis a, bit i
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            uint64_t bits = cut[a];
-            uint32_t p = a == 0 ? pos0 : (a == 1 ? pos1 : pos2);
-            if (p + (uint32_t)__popcll(bits) <= win || p >= win + kListCap) continue;
-            while (bits) {
-                const int b = __builtin_ctzll(bits);
-                bits &= bits - 1;
-                if (p >= win && p < win + kListCap)
-                    list[p - win] = ((uint32_t)rl.q << 20) | ((uint32_t)a << 18) | (uint32_t)(rl.w * 64 + b);
-                ++p;
-            }
-        }
-        wave_sync();
-        const uint32_t n = min((uint32_t)kListCap, total - win);
-        for (uint32_t e = lane; e < n; e += 64) {
-            const uint32_t ent = list[e];
-            const int64_t rr = r0 + (ent >> 20);
-            const int a = (int)((ent >> 18) & 3);
-            const int i = (int)(ent & 0x3FFFF);
-            const int jj = (int)(rr % ny), kk = (int)(rr / ny);
-            const uint64_t vid = vbase + win + e;
-            float p[3], nn[3];
-            fe_edge_point<Unit>(vol, g, iso, a, i, jj, kk, p, nn);
-            pts[3 * vid + 0] = p[0];
-            pts[3 * vid + 1] = p[1];
-            pts[3 * vid + 2] = p[2];
-            nrm[3 * vid + 0] = nn[0];
-            nrm[3 * vid + 1] = nn[1];
-            nrm[3 * vid + 2] = nn[2];
-        }
-        wave_sync();
-    }
-}
-
-/* ------------------------------------------------------------------ pass 4: triangles */
-constexpr int kTriMapCap = 512;     // triangle-map entries per wavefront and round (2 KB of LDS)
-
-/* The wavefront's triangles are one contiguous range of the output (its rows are consecutive).
- * Each lane lists the triangles of its word's active cells into an LDS map, in output order
- * (cells in increasing i, then case-table order); the wavefront then resolves the map one
- * triangle per lane, so the index stores of neighbouring lanes are contiguous. Vertex ids come
- * from the owning lane's 8 edge-combination cut words and their id bases, kept in LDS relative
- * to the wavefront's first vertex (the launch checks that these offsets fit 32 bits). */
-template <typename Index>
-__global__ __launch_bounds__(256) void k_fe_triangles(const uint64_t* __restrict__ mask, Geom g,
-                                                      const RowCounts* __restrict__ counts,
-                                                      const uint64_t* __restrict__ voff,
-                                                      const uint64_t* __restrict__ toff,
-                                                      Index* __restrict__ tris)
-{
-    __shared__ uint64_t s_cut[8][256];
-    __shared__ uint32_t s_base[8][256];
-    __shared__ uint64_t s_case[256];
-    __shared__ uint32_t s_map[4][kTriMapCap];
-    const int t = threadIdx.x;
-    const int lane = lane_id();
-    const int wv = t >> 6;
-    s_case[t] = kCaseEdges[t];
-    __syncthreads();
-    const RowLanes rl = row_lanes(g.W);
-    const int64_t R = g.ny * g.nz;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * rl.rpw;
-    if (r0 >= R) return;
-    const int64_t r = r0 + rl.q;
-    const int64_t j = r % g.ny, k = r / g.ny;
-    const bool cells = r < R && j < g.ny - 1 && k < g.nz - 1 && rl.w < g.W;
-    const uint32_t trow = cells ? counts[r].t : 0u;
-    if (__ballot(trow != 0) == 0) return;
-    const bool live = cells && trow != 0;
-    const LaneWords L = load_lane_words(mask, g, r, rl.w, live, true, true);
-    const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
-    uint64_t cw[8];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) cw[q] = (L.m[q] ^ shifted(L.m[q], L.n[q])) & eb;
-    cw[4] = (L.m[0] ^ L.m[1]) & vb;
-    cw[5] = (L.m[2] ^ L.m[3]) & vb;
-    cw[6] = (L.m[0] ^ L.m[2]) & vb;
-    cw[7] = (L.m[1] ^ L.m[3]) & vb;
+    const LaneWords L = load_lane_words(mask, g, r, rl.w, ok, has_y, has_z);
     const uint64_t v0 = voff[r0];
     const uint64_t t0 = toff[r0];
-    uint64_t cb[8] = {v0, v0, v0, v0, v0, v0, v0, v0};
-    if (live) {
-        const int64_t rows[4] = {r, r + 1, r + g.ny, r + g.ny + 1};
-        combo_bases(counts, voff, rows, cb);
-    }
+    const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
+
+    // ---------------- vertices: rows r0 .. r0 + rpw - 1, [x][y][z] cuts per row
+    {
+        const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb, has_y ? ((L.m[0] ^ L.m[1]) & vb) : 0ull,
+                                 has_z ? ((L.m[0] ^ L.m[2]) & vb) : 0ull};
+        const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
+        const uint32_t X = seg_sum(c0, rl.Wp), Y = seg_sum(c1, rl.Wp);
+        const uint32_t lane_tot = c0 + c1 + c2;
+        const uint32_t incl = wave_incl_scan_u32(lane_tot);
+        const uint32_t before_row = __shfl(incl - lane_tot, rl.q * rl.Wp);
+        const uint32_t pos0 = before_row + seg_excl_scan(c0, rl.Wp, rl.w);
+        const uint32_t pos1 = before_row + X + seg_excl_scan(c1, rl.Wp, rl.w);
+        const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        uint32_t* list = S.u.vlist;
+        for (uint32_t win = 0; win < total; win += kEmitVertCap) {
+            const uint32_t wend = win + kEmitVertCap;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        s_cut[q][t] = cw[q];
-        s_base[q][t] = (uint32_t)(cb[q] - v0) + seg_excl_scan((uint32_t)__popcll(cw[q]), rl.Wp, rl.w);
+            for (int a = 0; a < 3; ++a) {
+                uint64_t bits = cut[a];
+                uint32_t p = a == 0 ? pos0 : (a == 1 ? pos1 : pos2);
+                if (p + (uint32_t)__popcll(bits) <= win || p >= wend) continue;
+                while (bits && p < wend) {
+                    const int b = __builtin_ctzll(bits);
+                    bits &= bits - 1;
+                    if (p >= win) list[p - win] = ((uint32_t)rl.q << 20) | ((uint32_t)a << 18) | (uint32_t)(rl.w * 64 + b);
+                    ++p;
+                }
+            }
+            wave_sync();
+            const uint32_t n = min((uint32_t)kEmitVertCap, total - win);
+            for (uint32_t e = lane; e < n; e += 128) {
+                const bool two = e + 64 < n;
+                const uint32_t ent0 = list[e];
+                const uint32_t ent1 = two ? list[e + 64] : ent0;
+                float p0[3], n0[3], p1[3], n1[3];
+                {
+                    const int64_t rr = r0 + (ent0 >> 20);
+                    fe_edge_point<Unit>(vol, g, iso, (int)((ent0 >> 18) & 3), (int)(ent0 & 0x3FFFF), (int)(rr % ny),
+                                        (int)(rr / ny), p0, n0);
+                }
+                {
+                    const int64_t rr = r0 + (ent1 >> 20);
+                    fe_edge_point<Unit>(vol, g, iso, (int)((ent1 >> 18) & 3), (int)(ent1 & 0x3FFFF), (int)(rr % ny),
+                                        (int)(rr / ny), p1, n1);
+                }
+                const uint64_t vid = v0 + win + e;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    pts[3 * vid + c] = p0[c];
+                    nrm[3 * vid + c] = n0[c];
+                }
+                if (two) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        pts[3 * (vid + 64) + c] = p1[c];
+                        nrm[3 * (vid + 64) + c] = n1[c];
+                    }
+                }
+            }
+            wave_sync();
+        }
     }
+
+    // ---------------- triangles: cell rows r0 .. r0 + rpw - 1
+    const bool cells = ok && has_y && has_z;
     const CornerWords C = corner_words(L);
-    const uint64_t act = live ? active_cells(C, eb) : 0ull;
-    uint32_t nt = 0;
-    for (uint64_t a = act; a;) {
-        const int b = __builtin_ctzll(a);
-        a &= a - 1;
-        nt += (uint32_t)case_tri_count(s_case[case_of(C, b)]);
-    }
-    const uint32_t incl = wave_incl_scan_u32(nt);
-    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-    const uint32_t first = incl - nt;
-    uint32_t* map = s_map[wv];
-    for (uint32_t win = 0; win < total; win += kTriMapCap) {
-        const uint32_t wend = win + kTriMapCap;
-        if (nt && first < wend && first + nt > win) {
-            uint32_t p = first;
-            for (uint64_t a = act; a && p < wend;) {
-                const int b = __builtin_ctzll(a);
-                a &= a - 1;
-                const uint32_t cs = (uint32_t)case_of(C, b);
-                const int n = case_tri_count(s_case[cs]);
-                for (int tt = 0; tt < n; ++tt, ++p)
-                    if (p >= win && p < wend) map[p - win] = (uint32_t)lane | (uint32_t)b << 6 | (uint32_t)tt << 12 | cs << 15;
-            }
-        }
-        wave_sync();
-        const uint32_t n = min((uint32_t)kTriMapCap, total - win);
-        for (uint32_t e = lane; e < n; e += 64) {
-            const uint32_t ent = map[e];
-            const int owner = wv * 64 + (int)(ent & 63);
-            const int b = (int)((ent >> 6) & 63);
-            const int tt = (int)((ent >> 12) & 7);
-            const uint64_t packed = s_case[ent >> 15];
-            uint32_t id[3];
+    const uint64_t act = cells ? active_cells(C, eb) : 0ull;
+    const uint32_t ncell = (uint32_t)__popcll(act);
+    const uint32_t cincl = wave_incl_scan_u32(ncell);
+    const uint32_t ctotal = (uint32_t)__builtin_amdgcn_readlane((int)cincl, 63);
+    if (ctotal == 0) return;
+    {
+        uint64_t cw[8];
 #pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                const int ed = case_edge(packed, 3 * tt + s);
-                const int q = combo_of_edge(ed);
-                const int p = b + edge_dx(ed);
-                const uint64_t cwq = s_cut[q][owner];
-                const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
-                id[s] = s_base[q][owner] + (uint32_t)__popcll(below);
+        for (int q = 0; q < 4; ++q) cw[q] = (L.m[q] ^ shifted(L.m[q], L.n[q])) & eb;
+        cw[4] = (L.m[0] ^ L.m[1]) & vb;
+        cw[5] = (L.m[2] ^ L.m[3]) & vb;
+        cw[6] = (L.m[0] ^ L.m[2]) & vb;
+        cw[7] = (L.m[1] ^ L.m[3]) & vb;
+        uint64_t cb[8] = {v0, v0, v0, v0, v0, v0, v0, v0};
+        if (act) {
+            const int64_t rows[4] = {r, r + 1, r + g.ny, r + g.ny + 1};
+            combo_bases(counts, voff, rows, cb);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            S.cut[q][lane] = cw[q];
+            S.base[q][lane] = (uint32_t)(cb[q] - v0) + seg_excl_scan((uint32_t)__popcll(cw[q]), rl.Wp, rl.w);
+        }
+    }
+    uint32_t lo[8], hi[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        lo[2 * q] = (uint32_t)C.m[q];
+        hi[2 * q] = (uint32_t)(C.m[q] >> 32);
+        lo[2 * q + 1] = (uint32_t)C.s[q];
+        hi[2 * q + 1] = (uint32_t)(C.s[q] >> 32);
+    }
+    // corner order v0..v7 = m0 s0 s1 m1 m2 s2 s3 m3
+    const uint32_t clo[8] = {lo[0], lo[1], lo[3], lo[2], lo[4], lo[5], lo[7], lo[6]};
+    const uint32_t chi[8] = {hi[0], hi[1], hi[3], hi[2], hi[4], hi[5], hi[7], hi[6]};
+    const uint32_t cfirst = cincl - ncell;
+    uint64_t slot = t0;
+    for (uint32_t win = 0; win < ctotal; win += kEmitCellCap) {
+        const uint32_t wend = win + kEmitCellCap;
+        if (ncell && cfirst < wend && cfirst + ncell > win) {
+            uint32_t p = cfirst;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                uint32_t a = h ? (uint32_t)(act >> 32) : (uint32_t)act;
+                while (a && p < wend) {
+                    const int b = __builtin_ctz(a);
+                    a &= a - 1;
+                    if (p >= win) {
+                        const uint32_t cs = case_of_half(h ? chi : clo, b);
+                        S.u.t.cells[p - win] = (uint32_t)lane | (uint32_t)(b + 32 * h) << 6 | cs << 12;
+                    }
+                    ++p;
+                }
             }
-            store_tri<Index>(tris, t0 + win + e, v0 + id[0], v0 + id[1], v0 + id[2]);
         }
         wave_sync();
+        const uint32_t n = min((uint32_t)kEmitCellCap, ctotal - win);
+        for (uint32_t g0 = 0; g0 < n; g0 += 64) {
+            const uint32_t idx = g0 + (uint32_t)lane;
+            const uint32_t ent = idx < n ? S.u.t.cells[idx] : 0u;
+            const uint32_t nt = idx < n ? (uint32_t)case_tri_count(s_case[ent >> 12]) : 0u;
+            const uint32_t tincl = wave_incl_scan_u32(nt);
+            const uint32_t ttot = (uint32_t)__builtin_amdgcn_readlane((int)tincl, 63);
+            for (uint32_t tt = 0; tt < nt; ++tt) S.u.t.map[tincl - nt + tt] = ent | tt << 20;
+            wave_sync();
+            for (uint32_t e = lane; e < ttot; e += 64) {
+                const uint32_t me = S.u.t.map[e];
+                const int owner = (int)(me & 63);
+                const int b = (int)((me >> 6) & 63);
+                const uint64_t packed = s_case[(me >> 12) & 0xFF];
+                const int tt = (int)(me >> 20);
+                uint32_t id[3];
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    const int ed = case_edge(packed, 3 * tt + s);
+                    const int q = combo_of_edge(ed);
+                    const int p = b + edge_dx(ed);
+                    const uint64_t cwq = S.cut[q][owner];
+                    const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
+                    id[s] = S.base[q][owner] + (uint32_t)__popcll(below);
+                }
+                store_tri<Index>(tris, slot + e, v0 + id[0], v0 + id[1], v0 + id[2]);
+            }
+            slot += ttot;
+            wave_sync();
+        }
     }
 }
 
