@@ -171,9 +171,15 @@ struct RowState {
                g.org[2] == 0.0f && g.nx < (1 << 24) && g.ny < (1 << 24) && g.nz < (1 << 24);
     }
 
-    /* (row, word) lane kernels: rows of up to 4096 voxels, and the vertex ids a wavefront's
-     * triangles reference (rows r0 .. r0 + 64 + ny + 1) within 2^32 of its first one */
-    bool lane_rows() const { return g.W <= 64 && (g.ny + 130) * 3 * g.nx < (int64_t(1) << 32); }
+    /* (row, word) lane kernels: rows of up to 4096 voxels; the vertex ids a wavefront's
+     * triangles reference (rows r0 .. r0 + 64 + ny + 1) within 2^32 of its first one; the voxels
+     * its vertices read (planes k0 - 1 .. k0 + 64 / ny + 3) within 2^31 of plane k0 */
+    bool lane_rows() const
+    {
+        const int64_t planes = (63 + g.ny) / g.ny + 3;
+        return g.W <= 64 && (g.ny + 130) * 3 * g.nx < (int64_t(1) << 32) &&
+               planes * g.nx * g.ny < (int64_t(1) << 31);
+    }
     int rows_per_wave() const
     {
         int wp = 1;

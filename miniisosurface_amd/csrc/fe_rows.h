@@ -155,6 +155,36 @@ __device__ __forceinline__ uint64_t active_cells(const LaneWords& L, uint64_t eb
     return active_cells(corner_words(L), eb);
 }
 
+/* Triangles of the marching-cubes cases of a word's 64 cells, bit-sliced over the cells. For a
+ * cell with corners on both sides of the isovalue, the case table's triangle count
+ * (numTris, flyingEdges/util/MarchingCubesTables.h:23-41) equals V - 2F - 4S: V the corners
+ * >= iso, F the faces whose four corners are >= iso, S = 1 for the six-corner rings (both ends
+ * of a body diagonal below iso, the other six corners >= iso). Cells with all corners >= iso are
+ * masked out; cells with none contribute 0. */
+__device__ __forceinline__ uint32_t word_tri_count(const CornerWords& C, uint64_t eb)
+{
+    // corners v0..v7 of the cells (case_of's bit order)
+    const uint64_t v0 = C.m[0], v1 = C.s[0], v2 = C.s[1], v3 = C.m[1];
+    const uint64_t v4 = C.m[2], v5 = C.s[2], v6 = C.s[3], v7 = C.m[3];
+    const uint64_t a01 = v0 & v1, a23 = v2 & v3, a45 = v4 & v5, a67 = v6 & v7;
+    const uint64_t a03 = v0 & v3, a12 = v1 & v2, a56 = v5 & v6, a47 = v4 & v7;
+    const uint64_t M = eb & ~(a01 & a23 & a45 & a67);
+    uint32_t V = 0;
+    V += (uint32_t)__popcll(v0 & M) + (uint32_t)__popcll(v1 & M) + (uint32_t)__popcll(v2 & M) +
+         (uint32_t)__popcll(v3 & M);
+    V += (uint32_t)__popcll(v4 & M) + (uint32_t)__popcll(v5 & M) + (uint32_t)__popcll(v6 & M) +
+         (uint32_t)__popcll(v7 & M);
+    const uint32_t F = (uint32_t)__popcll(a01 & a23 & M) + (uint32_t)__popcll(a45 & a67 & M) +
+                       (uint32_t)__popcll(a01 & a45 & M) + (uint32_t)__popcll(a23 & a67 & M) +
+                       (uint32_t)__popcll(a03 & a47 & M) + (uint32_t)__popcll(a12 & a56 & M);
+    // body diagonals (0,6) (1,7) (2,4) (3,5): both ends set, and both ends clear
+    const uint64_t pd0 = v0 & v6, pd1 = v1 & v7, pd2 = v2 & v4, pd3 = v3 & v5;
+    const uint64_t p01 = pd0 & pd1, p23 = pd2 & pd3;
+    const uint32_t S = (uint32_t)__popcll(~(v0 | v6) & pd1 & p23 & eb) + (uint32_t)__popcll(~(v1 | v7) & pd0 & p23 & eb) +
+                       (uint32_t)__popcll(~(v2 | v4) & p01 & pd3 & eb) + (uint32_t)__popcll(~(v3 | v5) & p01 & pd2 & eb);
+    return V - 2 * F - 4 * S;
+}
+
 /* ------------------------------------------------------------------ pass 1b + 2 */
 __global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restrict__ mask, Geom g,
                                                        RowCounts* __restrict__ counts,
@@ -162,12 +192,9 @@ __global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restric
                                                        uint64_t* __restrict__ scan_state,
                                                        int64_t scan_state_words)
 {
-    __shared__ uint8_t s_ntri[256];
-    for (int c = threadIdx.x; c < 256; c += blockDim.x) s_ntri[c] = (uint8_t)case_tri_count(kCaseEdges[c]);
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < scan_state_words;
          t += (int64_t)gridDim.x * blockDim.x)
         scan_state[t] = 0;
-    __syncthreads();
 
     const RowLanes rl = row_lanes(g.W);
     const int64_t wave_id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -185,16 +212,7 @@ __global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restric
     // positions as unsigned: "none" is 0xFFFFFFFF for the minimum and 0 (= last + 1) for the maximum
     uint32_t first = trim ? (uint32_t)(64 * rl.w + __builtin_ctzll(trim)) : 0xFFFFFFFFu;
     uint32_t lastp1 = trim ? (uint32_t)(64 * rl.w + 64 - __builtin_clzll(trim)) : 0u;
-    uint32_t tc = 0;
-    if (has_y && has_z) {
-        const CornerWords C = corner_words(L);
-        uint64_t act = active_cells(C, eb);
-        while (act) {
-            const int b = __builtin_ctzll(act);
-            act &= act - 1;
-            tc += s_ntri[case_of(C, b)];
-        }
-    }
+    uint32_t tc = (has_y && has_z) ? word_tri_count(corner_words(L), eb) : 0u;
     const uint32_t xc = seg_sum((uint32_t)__popcll(xcut), rl.Wp);
     const uint32_t yc = seg_sum((uint32_t)__popcll(ycut), rl.Wp);
     const uint32_t zc = seg_sum((uint32_t)__popcll(zcut), rl.Wp);
@@ -209,33 +227,41 @@ __global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restric
 
 /* ------------------------------------------------------------------ pass 4: edge points */
 
-/* One gradient component, the reference's formula ((lower - upper) / run, one-sided at the
- * faces; flyingEdges/util/Image3D.cpp:116-188). With unit spacing the divisions by 1 and 2 are
- * exact and written as such. */
+/* One gradient component at the grid vertex *v, the reference's formula ((lower - upper) / run,
+ * one-sided at the faces; flyingEdges/util/Image3D.cpp:116-188). With unit spacing the divisions
+ * by 1 and 2 are exact and written as such. */
 template <bool Unit>
-__device__ __forceinline__ float grad_axis_at(const float* __restrict__ d, int64_t idx, int pos, int n, int64_t stride,
-                                              float s)
+__device__ __forceinline__ float grad_at(const float* __restrict__ v, int pos, int n, int stride, float s)
 {
-    float hi, lo;
     const bool first = pos == 0, last = pos == n - 1;
-    hi = last ? d[idx] : d[idx + stride];
-    lo = first ? d[idx] : d[idx - stride];
+    const float hi = v[last ? 0 : stride];
+    const float lo = v[first ? 0 : -stride];
     const bool one_sided = first || last;
     if (Unit) return one_sided ? (lo - hi) : (lo - hi) * 0.5f;
     return (lo - hi) / (one_sided ? s : 2.0f * s);
 }
 
-/* Point and normal of the crossing on the edge from grid vertex (i,j,k) along `axis`, with the
- * Flying Edges corner coordinates (fe_edge_corners); 32-bit indices, one 64-bit base. */
 template <bool Unit>
-__device__ __forceinline__ void fe_edge_point(const float* __restrict__ d, const Geom& g, float iso, int axis, int i,
+__device__ __forceinline__ float grad_interior(const float* __restrict__ v, int stride, float s)
+{
+    const float hi = v[stride], lo = v[-stride];
+    if (Unit) return (lo - hi) * 0.5f;
+    return (lo - hi) / (2.0f * s);
+}
+
+/* Point and normal of the crossing on the edge from grid vertex (i,j,k) along `axis`, with the
+ * Flying Edges corner coordinates (fe_edge_corners). `v` points at the voxel of (i,j,k); strides
+ * are 32-bit (the launch checks nx * ny < 2^31). Edges whose both ends are interior take a
+ * branch without the face tests; it computes the same formulas. */
+template <bool Unit>
+__device__ __forceinline__ void fe_edge_point(const float* __restrict__ v, const Geom& g, float iso, int axis, int i,
                                               int j, int k, float p[3], float nrm[3])
 {
     const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
-    const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
-    const int64_t base = (int64_t)i + (int64_t)j * sy + (int64_t)k * sz;
-    const int64_t step = axis == 0 ? sx : (axis == 1 ? sy : sz);
-    const float v0 = d[base], v1 = d[base + step];
+    const int sy = nx, sz = nx * ny;
+    const int step = axis == 0 ? 1 : (axis == 1 ? sy : sz);
+    const float* __restrict__ vb = v + step;
+    const float v0 = v[0], v1 = vb[0];
     const float w = (iso - v0) / (v1 - v0);
     float A[3], B[3];
     if (Unit) {
@@ -246,18 +272,140 @@ __device__ __forceinline__ void fe_edge_point(const float* __restrict__ d, const
         fe_edge_corners(g, axis, i, j, k, A, B);
     }
     const int ib = i + (axis == 0), jb = j + (axis == 1), kb = k + (axis == 2);
-    const int64_t bb = base + step;
-    const float ga0 = grad_axis_at<Unit>(d, base, i, nx, sx, g.sp[0]);
-    const float ga1 = grad_axis_at<Unit>(d, base, j, ny, sy, g.sp[1]);
-    const float ga2 = grad_axis_at<Unit>(d, base, k, nz, sz, g.sp[2]);
-    const float gb0 = grad_axis_at<Unit>(d, bb, ib, nx, sx, g.sp[0]);
-    const float gb1 = grad_axis_at<Unit>(d, bb, jb, ny, sy, g.sp[1]);
-    const float gb2 = grad_axis_at<Unit>(d, bb, kb, nz, sz, g.sp[2]);
+    float ga0, ga1, ga2, gb0, gb1, gb2;
+    if (i >= 1 && j >= 1 && k >= 1 && ib <= nx - 2 && jb <= ny - 2 && kb <= nz - 2) {
+        ga0 = grad_interior<Unit>(v, 1, g.sp[0]);
+        ga1 = grad_interior<Unit>(v, sy, g.sp[1]);
+        ga2 = grad_interior<Unit>(v, sz, g.sp[2]);
+        gb0 = grad_interior<Unit>(vb, 1, g.sp[0]);
+        gb1 = grad_interior<Unit>(vb, sy, g.sp[1]);
+        gb2 = grad_interior<Unit>(vb, sz, g.sp[2]);
+    } else {
+        ga0 = grad_at<Unit>(v, i, nx, 1, g.sp[0]);
+        ga1 = grad_at<Unit>(v, j, ny, sy, g.sp[1]);
+        ga2 = grad_at<Unit>(v, k, nz, sz, g.sp[2]);
+        gb0 = grad_at<Unit>(vb, ib, nx, 1, g.sp[0]);
+        gb1 = grad_at<Unit>(vb, jb, ny, sy, g.sp[1]);
+        gb2 = grad_at<Unit>(vb, kb, nz, sz, g.sp[2]);
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = A[c] + w * (B[c] - A[c]);
     nrm[0] = ga0 + w * (gb0 - ga0);
     nrm[1] = ga1 + w * (gb1 - ga1);
     nrm[2] = ga2 + w * (gb2 - ga2);
+}
+
+/* Unaligned (4-byte aligned) loads of 2, 3 or 4 consecutive voxels. */
+struct Vox2 { float x, y; };
+struct Vox3 { float x, y, z; };
+struct Vox4 { float x, y, z, w; };
+template <typename T>
+__device__ __forceinline__ T load_vox(const float* __restrict__ p)
+{
+    T t;
+    __builtin_memcpy(&t, p, sizeof(T));
+    return t;
+}
+
+/* interior central difference (lower - upper) / (2 h), exact as * 0.5 for unit spacing */
+template <bool Unit>
+__device__ __forceinline__ float central(float lo, float hi, float s)
+{
+    if (Unit) return (lo - hi) * 0.5f;
+    return (lo - hi) / (2.0f * s);
+}
+
+template <bool Unit>
+__device__ __forceinline__ void edge_finish(const Geom& g, int axis, int i, int j, int k, float w, const float ga[3],
+                                            const float gb[3], float p[3], float nrm[3])
+{
+    float A[3], B[3];
+    if (Unit) {
+        A[0] = (float)i; A[1] = (float)j; A[2] = (float)k;
+        B[0] = A[0]; B[1] = A[1]; B[2] = A[2];
+        B[axis] = A[axis] + 1.0f;
+    } else {
+        fe_edge_corners(g, axis, i, j, k, A, B);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        p[c] = A[c] + w * (B[c] - A[c]);
+        nrm[c] = ga[c] + w * (gb[c] - ga[c]);
+    }
+}
+
+/* x-edge (i,j,k)-(i+1,j,k): the 12 voxels in 5 loads (the row i-1..i+2, and the pairs i..i+1 in
+ * rows j-1, j+1 and planes k-1, k+1); edges touching a face take fe_edge_point. */
+template <bool Unit>
+__device__ __forceinline__ void vertex_x(const float* __restrict__ v, const Geom& g, float iso, int i, int j, int k,
+                                         float p[3], float nrm[3])
+{
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+    if (!(i >= 1 && i <= nx - 3 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2)) {
+        fe_edge_point<Unit>(v, g, iso, 0, i, j, k, p, nrm);
+        return;
+    }
+    const int sy = nx, sz = nx * ny;
+    const Vox4 r = load_vox<Vox4>(v - 1);
+    const Vox2 ym = load_vox<Vox2>(v - sy), yp = load_vox<Vox2>(v + sy);
+    const Vox2 zm = load_vox<Vox2>(v - sz), zp = load_vox<Vox2>(v + sz);
+    const float w = (iso - r.y) / (r.z - r.y);
+    const float ga[3] = {central<Unit>(r.x, r.z, g.sp[0]), central<Unit>(ym.x, yp.x, g.sp[1]),
+                         central<Unit>(zm.x, zp.x, g.sp[2])};
+    const float gb[3] = {central<Unit>(r.y, r.w, g.sp[0]), central<Unit>(ym.y, yp.y, g.sp[1]),
+                         central<Unit>(zm.y, zp.y, g.sp[2])};
+    edge_finish<Unit>(g, 0, i, j, k, w, ga, gb, p, nrm);
+}
+
+/* y- or z-edge from (i,j,k) along `axis` (1 or 2): the voxels i-1..i+1 of both end rows in two
+ * loads, the two further voxels along the edge, and the four across it (axis c, the third). */
+template <bool Unit>
+__device__ __forceinline__ void vertex_yz(const float* __restrict__ v, const Geom& g, float iso, int axis, int i,
+                                          int j, int k, float p[3], float nrm[3])
+{
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+    const bool ay = axis == 1;
+    const int pa = ay ? j : k, na = ay ? ny : nz, pc = ay ? k : j, nc = ay ? nz : ny;
+    if (!(i >= 1 && i <= nx - 2 && pa >= 1 && pa <= na - 3 && pc >= 1 && pc <= nc - 2)) {
+        fe_edge_point<Unit>(v, g, iso, axis, i, j, k, p, nrm);
+        return;
+    }
+    const int sy = nx, sz = nx * ny;
+    const int sa = ay ? sy : sz, sc = ay ? sz : sy;
+    const float* __restrict__ vb = v + sa;
+    const Vox3 xa = load_vox<Vox3>(v - 1), xb = load_vox<Vox3>(vb - 1);
+    const float am = v[-sa], bp = vb[sa];
+    const float cam = v[-sc], cap = v[sc], cbm = vb[-sc], cbp = vb[sc];
+    const float w = (iso - xa.y) / (xb.y - xa.y);
+    const float spa = ay ? g.sp[1] : g.sp[2], spc = ay ? g.sp[2] : g.sp[1];
+    const float gax = central<Unit>(xa.x, xa.z, g.sp[0]), gbx = central<Unit>(xb.x, xb.z, g.sp[0]);
+    const float gaa = central<Unit>(am, xb.y, spa), gba = central<Unit>(xa.y, bp, spa);
+    const float gac = central<Unit>(cam, cap, spc), gbc = central<Unit>(cbm, cbp, spc);
+    const float ga[3] = {gax, ay ? gaa : gac, ay ? gac : gaa};
+    const float gb[3] = {gbx, ay ? gba : gbc, ay ? gbc : gba};
+    edge_finish<Unit>(g, axis, i, j, k, w, ga, gb, p, nrm);
+}
+
+/* A vertex-list entry: output position (wavefront-relative) << 14 | axis << 12 | lane << 6 | bit,
+ * the lane (q, w) and bit b naming row q of the wavefront and i = 64 w + b. The row is row j0 + q
+ * of plane k0, carried into the following planes. */
+template <bool Unit>
+__device__ __forceinline__ void emit_vertex(const float* __restrict__ plane, const Geom& g, float iso, uint32_t ent,
+                                            int j0, int k0, int wp_log2, bool xgroup, float p[3], float nrm[3])
+{
+    const int ny = (int)g.ny;
+    const int src = (int)((ent >> 6) & 63);
+    int j = j0 + (src >> wp_log2), dk = 0;
+    while (j >= ny) {
+        j -= ny;
+        ++dk;
+    }
+    const int i = ((src & ((1 << wp_log2) - 1)) << 6) | (int)(ent & 63);
+    const uint32_t off = (uint32_t)dk * (uint32_t)(g.nx * g.ny) + (uint32_t)j * (uint32_t)g.nx + (uint32_t)i;
+    if (xgroup)
+        vertex_x<Unit>(plane + off, g, iso, i, j, k0 + dk, p, nrm);
+    else
+        vertex_yz<Unit>(plane + off, g, iso, (int)((ent >> 12) & 3), i, j, k0 + dk, p, nrm);
 }
 
 /* ------------------------------------------------------------------ pass 4: fused emit */
@@ -322,6 +470,11 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
     const uint64_t v0 = voff[r0];
     const uint64_t t0 = toff[r0];
     const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
+    // the wavefront's first row (j0, k0); voxel offsets below are 32-bit from plane k0
+    const int64_t k0 = r0 / g.ny;
+    const int j0 = (int)(r0 - k0 * g.ny);
+    const uint32_t sz = (uint32_t)(g.nx * g.ny);
+    const float* __restrict__ plane = vol + k0 * (int64_t)sz;
 
     // ---------------- vertices: rows r0 .. r0 + rpw - 1, [x][y][z] cuts per row
     {
@@ -336,53 +489,63 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
         const uint32_t pos1 = before_row + X + seg_excl_scan(c1, rl.Wp, rl.w);
         const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
         const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        // two lists, x-edges and then y/z-edges, so that the lanes of one step take the same
+        // load pattern; each entry carries its output position
+        const uint32_t xincl = wave_incl_scan_u32(c0);
+        const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl, 63);
+        const uint32_t xfirst = xincl - c0;
+        const uint32_t yzfirst = (incl - lane_tot) - xfirst;
+        const int wp_log2 = __builtin_ctz((unsigned)rl.Wp);
         uint32_t* list = S.u.vlist;
-        for (uint32_t win = 0; win < total; win += kEmitVertCap) {
-            const uint32_t wend = win + kEmitVertCap;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                uint64_t bits = cut[a];
-                uint32_t p = a == 0 ? pos0 : (a == 1 ? pos1 : pos2);
-                if (p + (uint32_t)__popcll(bits) <= win || p >= wend) continue;
-                while (bits && p < wend) {
-                    const int b = __builtin_ctzll(bits);
-                    bits &= bits - 1;
-                    if (p >= win) list[p - win] = ((uint32_t)rl.q << 20) | ((uint32_t)a << 18) | (uint32_t)(rl.w * 64 + b);
-                    ++p;
-                }
-            }
-            wave_sync();
-            const uint32_t n = min((uint32_t)kEmitVertCap, total - win);
-            for (uint32_t e = lane; e < n; e += 128) {
-                const bool two = e + 64 < n;
-                const uint32_t ent0 = list[e];
-                const uint32_t ent1 = two ? list[e + 64] : ent0;
-                float p0[3], n0[3], p1[3], n1[3];
-                {
-                    const int64_t rr = r0 + (ent0 >> 20);
-                    fe_edge_point<Unit>(vol, g, iso, (int)((ent0 >> 18) & 3), (int)(ent0 & 0x3FFFF), (int)(rr % ny),
-                                        (int)(rr / ny), p0, n0);
-                }
-                {
-                    const int64_t rr = r0 + (ent1 >> 20);
-                    fe_edge_point<Unit>(vol, g, iso, (int)((ent1 >> 18) & 3), (int)(ent1 & 0x3FFFF), (int)(rr % ny),
-                                        (int)(rr / ny), p1, n1);
-                }
-                const uint64_t vid = v0 + win + e;
+        for (int grp = 0; grp < 2; ++grp) {
+            const uint32_t gtotal = grp ? total - xtotal : xtotal;
+            const uint32_t gfirst = grp ? yzfirst : xfirst;
+            for (uint32_t win = 0; win < gtotal; win += kEmitVertCap) {
+                const uint32_t wend = win + kEmitVertCap;
+                uint32_t idx = gfirst;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    pts[3 * vid + c] = p0[c];
-                    nrm[3 * vid + c] = n0[c];
+                for (int a = grp; a < (grp ? 3 : 1); ++a) {
+                    uint64_t bits = cut[a];
+                    uint32_t p = a == 0 ? pos0 : (a == 1 ? pos1 : pos2);
+                    if (idx + (uint32_t)__popcll(bits) <= win || idx >= wend) {
+                        idx += (uint32_t)__popcll(bits);
+                        continue;
+                    }
+                    while (bits && idx < wend) {
+                        const int b = __builtin_ctzll(bits);
+                        bits &= bits - 1;
+                        if (idx >= win) list[idx - win] = p << 14 | (uint32_t)a << 12 | (uint32_t)lane << 6 | (uint32_t)b;
+                        ++idx;
+                        ++p;
+                    }
+                    idx += (uint32_t)__popcll(bits);
                 }
-                if (two) {
+                wave_sync();
+                const uint32_t n = min((uint32_t)kEmitVertCap, gtotal - win);
+                for (uint32_t e = lane; e < n; e += 128) {
+                    const bool two = e + 64 < n;
+                    const uint32_t ent0 = list[e];
+                    const uint32_t ent1 = two ? list[e + 64] : ent0;
+                    float p0[3], n0[3], p1[3], n1[3];
+                    emit_vertex<Unit>(plane, g, iso, ent0, j0, (int)k0, wp_log2, grp == 0, p0, n0);
+                    emit_vertex<Unit>(plane, g, iso, ent1, j0, (int)k0, wp_log2, grp == 0, p1, n1);
+                    const uint64_t vid0 = v0 + (ent0 >> 14), vid1 = v0 + (ent1 >> 14);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        pts[3 * (vid + 64) + c] = p1[c];
-                        nrm[3 * (vid + 64) + c] = n1[c];
+                        pts[3 * vid0 + c] = p0[c];
+                        nrm[3 * vid0 + c] = n0[c];
+                    }
+                    if (two) {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            pts[3 * vid1 + c] = p1[c];
+                            nrm[3 * vid1 + c] = n1[c];
+                        }
                     }
                 }
+                wave_sync();
             }
-            wave_sync();
         }
     }
 

@@ -75,6 +75,25 @@ def test_case_table_invariants():
         assert set(table[255 - c]) == cut
 
 
+def test_triangle_count_closed_form(oracle_mod):
+    """The GPU count pass sums triangles per 64-cell word without a table lookup, as
+    V - 2F - 4S per cell (V corners >= iso, F faces with 4 such corners, S the six-corner
+    rings around a body diagonal). It must equal the oracle's numTris for every mixed case."""
+    import ctypes
+    L = oracle_mod.lib()
+    L.ora_tables_init()
+    num = np.frombuffer((ctypes.c_ubyte * 256).in_dll(L, "ora_num_tris"), dtype=np.uint8).astype(int)
+    faces = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4), (1, 2, 6, 5)]
+    diagonals = [(0, 6), (1, 7), (2, 4), (3, 5)]
+    for c in range(1, 255):
+        b = [(c >> q) & 1 for q in range(8)]
+        V = sum(b)
+        F = sum(all(b[q] for q in f) for f in faces)
+        S = sum(V == 6 and not b[p] and not b[q] for p, q in diagonals)
+        assert num[c] == V - 2 * F - 4 * S, c
+    assert num[0] == 0 and num[255] == 0
+
+
 def test_product_table_equals_oracle_table(oracle_mod):
     import ctypes
     L = oracle_mod.lib()
