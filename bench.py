@@ -55,6 +55,64 @@ def cpu_baseline(vol_np, iso, threads):
                       f"timer = reference ctor..moveOutput ({t:.3f} s)"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fe_hbm_traffic_1024_iso0.5.txt")
+CALIB_BYTES = 1 << 30     # the plain copy that calibrates the FETCH_SIZE / WRITE_SIZE counters
+
+
+def _pmc_rows(path):
+    import csv
+    for dirpath, _, files in os.walk(path):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                yield from csv.DictReader(open(os.path.join(dirpath, f)))
+
+
+def summarize_pmc(fetch_dir, write_dir, out_path, command):
+    """Turns two rocprofv3 --pmc runs of `bench.py --calibrate` (FETCH_SIZE, then WRITE_SIZE,
+    KiB per dispatch) into per-kernel HBM bytes per launch. Each counter is scaled by the ratio of
+    the known bytes of a plain 1 GiB device copy, run in the same process, to what the counter
+    reads for that copy."""
+    per = {}
+    for counter, path in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        calib = 0.0
+        sums = {}
+        rows = sorted((r for r in _pmc_rows(path) if r["Counter_Name"] == counter), key=lambda r: int(r["Dispatch_Id"]))
+        for r in rows:
+            name, val = r["Kernel_Name"], float(r["Counter_Value"]) * 1024.0
+            if "miso::" not in name and sums == {} and ("copyBuffer" in name or "direct_copy" in name):
+                calib = val      # the last device copy before the first extraction: the calibration copy
+            if "miso::" in name:
+                key = name.split("(")[0].replace("void ", "")
+                n, t = sums.get(key, (0, 0.0))
+                sums[key] = (n + 1, t + val)
+        if calib <= 0:
+            raise SystemExit(f"no calibration copy found in {path}")
+        factor = CALIB_BYTES / calib
+        for key, (n, t) in sums.items():
+            per.setdefault(key, {})[counter] = (t / n * factor, t / n, factor)
+    lines = [f"# HBM traffic per launch, rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy",
+             f"# command: {command}"]
+    for key, d in sorted(per.items()):
+        fb, fr, ff = d.get("FETCH_SIZE", (0, 0, 0))
+        wb, wr, wf = d.get("WRITE_SIZE", (0, 0, 0))
+        lines.append(f"{key}: fetch_bytes={fb:.0f} write_bytes={wb:.0f} traffic_bytes={fb + wb:.0f} "
+                     f"(raw fetch {fr:.0f} x {ff:.3f}, raw write {wr:.0f} x {wf:.3f})")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary, or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith(kernel + "<") or line.startswith(kernel + ":"):
+            for tok in line.split():
+                if tok.startswith("traffic_bytes="):
+                    return float(tok.split("=")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,7 +122,15 @@ def main():
     ap.add_argument("--iso", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="run one plain 1 GiB device copy first (calibrates rocprofv3 byte counters)")
+    ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT"),
+                    help="write the per-kernel HBM traffic summary of two rocprofv3 --pmc runs and exit")
+    ap.add_argument("--pmc-command", default="", help="the profiled command, recorded in the summary")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, command=args.pmc_command)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
               file=sys.stderr)
@@ -87,6 +153,13 @@ def main():
     N = n ** 3
 
     algo = m.FlyingEdgesAlgorithm(m.DeviceVolume.from_tensor(vol), args.iso, stream=stream)
+    if args.calibrate:
+        src = torch.empty(CALIB_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def run_steps(k):
         ms = np.zeros(4)
@@ -109,16 +182,22 @@ def main():
     R = n * n
     W = (n + 63) // 64
     mask_b = 8 * W * R
+    rows = algo.rows()
+    # grid rows holding a vertex: pass 4 must read those rows of the volume (the gradient stencil
+    # reaches their neighbours, which are almost always active rows themselves at this density)
+    active_rows = int(np.count_nonzero(rows["xcount"].astype(np.int64) + rows["ycount"] + rows["zcount"]))
     per_kernel = {
-        "fe_classify": {"ms": pass_ms[0], "bytes": 4 * N + mask_b},
-        "fe_count": {"ms": pass_ms[1], "bytes": mask_b + 24 * R},
-        "fe_scan": {"ms": pass_ms[2], "bytes": 16 * R + 16 * R},
-        "fe_emit": {"ms": pass_ms[3], "bytes": 24 * V + (bits // 8) * 3 * T + mask_b},
+        "miso::k_fe_classify": {"ms": pass_ms[0], "bytes": 4 * N + mask_b},
+        "miso::k_fe_count_rows": {"ms": pass_ms[1], "bytes": mask_b + 24 * R},
+        "miso::k_fe_scan": {"ms": pass_ms[2], "bytes": 16 * R + 16 * R},
+        "miso::k_fe_emit_rows": {"ms": pass_ms[3],
+                                 "bytes": 4 * n * active_rows + 24 * V + (bits // 8) * 3 * T + mask_b},
     }
     dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
     dk = per_kernel[dom]
     achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
     kern_total = float(pass_ms.sum())
+    traffic = read_traffic(dom) if (n == 1024 and args.iso == 0.5) else None
 
     # the sparse isovalue of the same config
     sparse = None
@@ -148,7 +227,9 @@ def main():
         "config": {"workload": f"flying edges, {n}^3 float32 3-octave value noise (seed 0), iso {args.iso}",
                    "nverts": V, "ntris": T, "index_bits": bits},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else int(traffic),
+                     "alg_bytes": int(dk["bytes"]), "kernel_ms": round(float(dk["ms"]), 4)},
         "extraction": {"bytes_alg": b_alg, "kernel_ms": round(kern_total, 4),
                        "hbm_frac_kernels": round(b_alg / (kern_total * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4),
