@@ -109,6 +109,35 @@ int miso_mc_device_output(miso_mc* h, const float** points, const float** normal
 int miso_mc_time_ms(const miso_mc* h, float* ms);
 void miso_mc_destroy(miso_mc* h);
 
+/* ---------------------------------------------------------------- batched Flying Edges */
+/* No counterpart in the reference, which extracts one volume per process
+ * (flyingEdges/serial/main.cpp:110-158). A batch is a fixed list of jobs -- device-resident
+ * volumes of any shape (dimensions >= 2, rows of at most 4096 voxels), each with its own
+ * isovalue; several jobs may share one volume -- extracted by one persistent launch per pass,
+ * replayed from captured HIP graphs. Each job's result is bit-identical to miso_fe_run on that
+ * volume and isovalue; its triangles index its own vertices from 0 (32-bit). */
+typedef struct miso_batch miso_batch;
+
+int miso_batch_create(const miso_volume* volumes, const float* isovals, int64_t njobs, void* stream,
+                      miso_batch** out);
+/* passes 1-4 for every job; synchronous */
+int miso_batch_run(miso_batch* b);
+int miso_batch_totals(const miso_batch* b, uint64_t* nverts, uint64_t* ntris);
+int miso_batch_job_counts(const miso_batch* b, int64_t job, uint64_t* nverts, uint64_t* ntris);
+/* device pointers into the batch's shared output, valid until the next run or destroy */
+int miso_batch_job_output(const miso_batch* b, int64_t job, const float** points, const float** normals,
+                          const uint32_t** triangles);
+/* host copy of one job's mesh, triangles widened to 64-bit (util::TriangleMesh's size_t) */
+int miso_batch_copy_out(miso_batch* b, int64_t job, float* points, float* normals, uint64_t* triangles);
+/* launch shape of the persistent kernels: resident blocks per CU and work items per ticket */
+int miso_batch_set_tuning(miso_batch* b, int blocks_per_cu, int chunk);
+int miso_batch_get_tuning(const miso_batch* b, int* blocks_per_cu, int* chunk);
+/* times every shape of a small grid on this batch (reps runs each), keeps the fastest */
+int miso_batch_autotune(miso_batch* b, int reps, float* best_ms);
+/* device time of the last run, passes 1-4 */
+int miso_batch_time_ms(const miso_batch* b, float* ms);
+void miso_batch_destroy(miso_batch* b);
+
 /* ---------------------------------------------------------------- library */
 const char* miso_last_error(void);
 /* "miniiso <version> gfx950" */

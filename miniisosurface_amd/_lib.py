@@ -71,12 +71,22 @@ def lib():
         "miso_mc_copy_out": [P, P, P, P],
         "miso_mc_device_output": [P, PP, PP, PP, ctypes.POINTER(I)],
         "miso_mc_time_ms": [P, ctypes.POINTER(F)],
+        "miso_batch_create": [ctypes.POINTER(Volume), ctypes.POINTER(F), ctypes.c_int64, P, PP],
+        "miso_batch_run": [P],
+        "miso_batch_totals": [P, U64P, U64P],
+        "miso_batch_job_counts": [P, ctypes.c_int64, U64P, U64P],
+        "miso_batch_job_output": [P, ctypes.c_int64, PP, PP, PP],
+        "miso_batch_copy_out": [P, ctypes.c_int64, P, P, P],
+        "miso_batch_set_tuning": [P, I, I],
+        "miso_batch_get_tuning": [P, ctypes.POINTER(I), ctypes.POINTER(I)],
+        "miso_batch_autotune": [P, I, ctypes.POINTER(F)],
+        "miso_batch_time_ms": [P, ctypes.POINTER(F)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = I
-    for name in ("miso_fe_destroy", "miso_mc_destroy"):
+    for name in ("miso_fe_destroy", "miso_mc_destroy", "miso_batch_destroy"):
         getattr(L, name).argtypes = [P]
         getattr(L, name).restype = None
     L.miso_last_error.restype = ctypes.c_char_p

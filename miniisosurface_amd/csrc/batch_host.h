@@ -1,0 +1,423 @@
+/*
+ * batch_host.h -- host side of the batched Flying Edges entry points (miso_batch_* in
+ * include/miniiso.h). Included once, by capi.hip, after its helpers.
+ *
+ * A batch is a fixed list of jobs (device volume, isovalue). Passes 1-3 and the per-job count
+ * gather run as one captured hipGraph; pass 4 as a second one, or both as a single graph once
+ * the output capacity is known from an earlier run. The output of all jobs shares one buffer of
+ * points, one of normals and one of 32-bit triangle indices.
+ */
+
+struct miso_batch {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int njobs = 0;
+    std::vector<BatchJob> jobs;
+    std::vector<int64_t> cls_prefix, lane_prefix;    // work items per job, prefix sums
+    BatchJob* d_jobs = nullptr;
+    int64_t* d_cls_prefix = nullptr;
+    int64_t* d_lane_prefix = nullptr;
+    int64_t R = 0, mask_words = 0, ntiles = 0;
+    uint64_t* d_mask = nullptr;
+    RowCounts* d_counts = nullptr;
+    RowTrim* d_trims = nullptr;
+    uint64_t* d_voff = nullptr;
+    uint64_t* d_toff = nullptr;
+    uint64_t* d_state = nullptr;      // [0..2] queue tickets, [3] totals V, [4] totals T, [5..] scan state
+    int64_t state_words = 0;
+    uint64_t* d_jobout = nullptr;     // 4 per job: first vertex, first triangle, vertices, triangles
+    uint64_t* h_jobout = nullptr;     // pinned copy, then the two totals
+    float* d_pts = nullptr;
+    float* d_nrm = nullptr;
+    uint32_t* d_tris = nullptr;
+    uint64_t cap_v = 0, cap_t = 0;
+    int blocks_per_cu = 4, chunk = 4, num_cus = 256;
+    hipGraphExec_t g_scan = nullptr, g_emit = nullptr, g_all = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    bool ran = false;
+
+    uint64_t* tickets() { return d_state; }
+    uint64_t* totals() { return d_state + 3; }
+    uint64_t* scan_state() { return d_state + 5; }
+};
+
+namespace {
+
+bool geom_unit(const Geom& g)
+{
+    return g.sp[0] == 1.0f && g.sp[1] == 1.0f && g.sp[2] == 1.0f && g.org[0] == 0.0f && g.org[1] == 0.0f &&
+           g.org[2] == 0.0f && g.nx < (1 << 24) && g.ny < (1 << 24) && g.nz < (1 << 24);
+}
+
+bool geom_lane_rows(const Geom& g)
+{
+    const int64_t planes = (63 + g.ny) / g.ny + 3;
+    return g.W <= 64 && (g.ny + 130) * 3 * g.nx < (int64_t(1) << 32) && planes * g.nx * g.ny < (int64_t(1) << 31);
+}
+
+int64_t geom_rows_per_wave(const Geom& g)
+{
+    int64_t wp = 1;
+    while (wp < g.W) wp <<= 1;
+    return 64 / wp;
+}
+
+void destroy_graphs(miso_batch* b)
+{
+    if (b->g_scan) (void)hipGraphExecDestroy(b->g_scan);
+    if (b->g_emit) (void)hipGraphExecDestroy(b->g_emit);
+    if (b->g_all) (void)hipGraphExecDestroy(b->g_all);
+    b->g_scan = b->g_emit = b->g_all = nullptr;
+}
+
+unsigned batch_grid(const miso_batch* b, int64_t nitems)
+{
+    const int64_t want = (int64_t)b->num_cus * b->blocks_per_cu;
+    const int64_t need = (nitems + b->chunk - 1) / b->chunk;
+    return (unsigned)std::max<int64_t>(1, std::min(want, need));
+}
+
+int enqueue_scan(miso_batch* b)
+{
+    const int64_t ncls = b->cls_prefix.back(), nlane = b->lane_prefix.back();
+    hipLaunchKernelGGL(k_batch_zero, dim3((unsigned)std::min<int64_t>(1024, (b->state_words + 255) / 256)), dim3(256),
+                       0, b->stream, b->d_state, b->state_words);
+    hipLaunchKernelGGL(k_batch_classify, dim3(batch_grid(b, ncls)), dim3(256), 0, b->stream,
+                       (const BatchJob*)b->d_jobs, (const int64_t*)b->d_cls_prefix, b->njobs, ncls, b->chunk, b->d_mask,
+                       b->tickets() + 0);
+    hipLaunchKernelGGL(k_batch_count, dim3(batch_grid(b, nlane)), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
+                       (const int64_t*)b->d_lane_prefix, b->njobs, nlane, b->chunk, (const uint64_t*)b->d_mask,
+                       b->d_counts, b->d_trims, b->tickets() + 1);
+    hipLaunchKernelGGL(k_fe_scan, dim3((unsigned)b->ntiles), dim3(256), 0, b->stream, (const RowCounts*)b->d_counts,
+                       b->R, b->d_voff, b->d_toff, b->scan_state(), b->ntiles, b->totals());
+    hipLaunchKernelGGL(k_batch_job_counts, dim3((unsigned)((b->njobs + 255) / 256)), dim3(256), 0, b->stream,
+                       (const BatchJob*)b->d_jobs, b->njobs, b->R, (const uint64_t*)b->d_voff,
+                       (const uint64_t*)b->d_toff, (const uint64_t*)b->totals(), b->d_jobout);
+    HIPCALL(hipGetLastError());
+    HIPCALL(hipMemcpyAsync(b->h_jobout, b->d_jobout, 4 * (size_t)b->njobs * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIPCALL(hipMemcpyAsync(b->h_jobout + 4 * b->njobs, b->totals(), 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           b->stream));
+    return MISO_OK;
+}
+
+int enqueue_emit(miso_batch* b)
+{
+    const int64_t nlane = b->lane_prefix.back();
+    hipLaunchKernelGGL(k_batch_emit, dim3(batch_grid(b, nlane)), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
+                       (const int64_t*)b->d_lane_prefix, b->njobs, nlane, b->chunk, (const uint64_t*)b->d_mask,
+                       (const RowCounts*)b->d_counts, (const uint64_t*)b->d_voff, (const uint64_t*)b->d_toff,
+                       (const uint64_t*)b->d_jobout, (const uint64_t*)b->totals(), b->cap_v, b->cap_t, b->d_pts,
+                       b->d_nrm, b->d_tris, b->tickets() + 2);
+    HIPCALL(hipGetLastError());
+    return MISO_OK;
+}
+
+/* captures what `fn` enqueues on the batch stream into an executable graph */
+template <class Fn>
+int capture(miso_batch* b, hipGraphExec_t* out, Fn fn)
+{
+    hipGraph_t graph = nullptr;
+    HIPCALL(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = fn();
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(MISO_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(MISO_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return MISO_OK;
+}
+
+int batch_capture_emit(miso_batch* b)
+{
+    if (b->g_emit) (void)hipGraphExecDestroy(b->g_emit);
+    if (b->g_all) (void)hipGraphExecDestroy(b->g_all);
+    b->g_emit = b->g_all = nullptr;
+    if (int rc = capture(b, &b->g_emit, [&] { return enqueue_emit(b); })) return rc;
+    return capture(b, &b->g_all, [&] {
+        if (int rc = enqueue_scan(b)) return rc;
+        return enqueue_emit(b);
+    });
+}
+
+int batch_reserve(miso_batch* b, uint64_t v, uint64_t t)
+{
+    if (v <= b->cap_v && t <= b->cap_t && b->d_pts) return MISO_OK;
+    HIPCALL(hipStreamSynchronize(b->stream));
+    const uint64_t nv = std::max(v, b->cap_v), nt = std::max(t, b->cap_t);
+    dev_free(b->d_pts);
+    dev_free(b->d_nrm);
+    dev_free(b->d_tris);
+    b->cap_v = b->cap_t = 0;
+    if (int rc = dev_alloc(&b->d_pts, (size_t)(3 * nv))) return rc;
+    if (int rc = dev_alloc(&b->d_nrm, (size_t)(3 * nv))) return rc;
+    if (int rc = dev_alloc(&b->d_tris, (size_t)(3 * nt))) return rc;
+    b->cap_v = nv;
+    b->cap_t = nt;
+    return batch_capture_emit(b);
+}
+
+void batch_release(miso_batch* b)
+{
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    destroy_graphs(b);
+    dev_free(b->d_jobs);
+    dev_free(b->d_cls_prefix);
+    dev_free(b->d_lane_prefix);
+    dev_free(b->d_mask);
+    dev_free(b->d_counts);
+    dev_free(b->d_trims);
+    dev_free(b->d_voff);
+    dev_free(b->d_toff);
+    dev_free(b->d_state);
+    dev_free(b->d_jobout);
+    dev_free(b->d_pts);
+    dev_free(b->d_nrm);
+    dev_free(b->d_tris);
+    if (b->h_jobout) (void)hipHostFree(b->h_jobout);
+    b->h_jobout = nullptr;
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    b->ev0 = b->ev1 = nullptr;
+    if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
+    b->stream = nullptr;
+}
+
+int batch_init(miso_batch* b, const miso_volume* vols, const float* isovals, int64_t njobs, void* user_stream)
+{
+    if (njobs <= 0 || njobs > (1 << 24)) return fail(MISO_ERR_ARG, "job count %lld out of range", (long long)njobs);
+    if (!vols || !isovals) return fail(MISO_ERR_ARG, "volumes or isovalues NULL");
+    b->njobs = (int)njobs;
+    b->jobs.resize(njobs);
+    b->cls_prefix.assign(njobs + 1, 0);
+    b->lane_prefix.assign(njobs + 1, 0);
+    int64_t R = 0, mw = 0;
+    for (int64_t j = 0; j < njobs; ++j) {
+        const miso_volume* v = &vols[j];
+        if (int rc = check_volume(v)) return rc;
+        if (!v->on_device) return fail(MISO_ERR_ARG, "job %lld: batch volumes must be device-resident", (long long)j);
+        BatchJob& J = b->jobs[j];
+        J.g = make_geom(v);
+        if (!has_cells(J.g)) return fail(MISO_ERR_ARG, "job %lld: dimensions below 2", (long long)j);
+        if (!geom_lane_rows(J.g)) return fail(MISO_ERR_ARG, "job %lld: volume too large for a batch", (long long)j);
+        if (3 * J.g.nx * J.g.ny * J.g.nz >= (int64_t(1) << 32))
+            return fail(MISO_ERR_ARG, "job %lld: vertex ids may exceed 32 bits", (long long)j);
+        J.vol = v->data;
+        J.iso = isovals[j];
+        J.unit = geom_unit(J.g) ? 1 : 0;
+        J.row_base = R;
+        J.mask_base = mw;
+        const int64_t rows = J.g.ny * J.g.nz;
+        R += rows;
+        mw += rows * J.g.W;
+        const int64_t per_block = 4 * geom_rows_per_wave(J.g);
+        b->cls_prefix[j + 1] = b->cls_prefix[j] + (rows + kRowsPerBlock - 1) / kRowsPerBlock;
+        b->lane_prefix[j + 1] = b->lane_prefix[j] + (rows + per_block - 1) / per_block;
+    }
+    b->R = R;
+    b->mask_words = mw;
+    b->ntiles = (R + kScanTile - 1) / kScanTile;
+    b->state_words = 5 + 1 + 2 * b->ntiles;
+    if (user_stream) {
+        b->stream = (hipStream_t)user_stream;
+    } else {
+        HIPCALL(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->own_stream = true;
+    }
+    int dev = 0;
+    HIPCALL(hipGetDevice(&dev));
+    HIPCALL(hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int rc = 0;
+    if ((rc = dev_alloc(&b->d_jobs, (size_t)njobs))) return rc;
+    if ((rc = dev_alloc(&b->d_cls_prefix, (size_t)njobs + 1))) return rc;
+    if ((rc = dev_alloc(&b->d_lane_prefix, (size_t)njobs + 1))) return rc;
+    if ((rc = dev_alloc(&b->d_mask, (size_t)mw))) return rc;
+    if ((rc = dev_alloc(&b->d_counts, (size_t)R))) return rc;
+    if ((rc = dev_alloc(&b->d_trims, (size_t)R))) return rc;
+    if ((rc = dev_alloc(&b->d_voff, (size_t)R))) return rc;
+    if ((rc = dev_alloc(&b->d_toff, (size_t)R))) return rc;
+    if ((rc = dev_alloc(&b->d_state, (size_t)b->state_words))) return rc;
+    if ((rc = dev_alloc(&b->d_jobout, 4 * (size_t)njobs))) return rc;
+    HIPCALL(hipHostMalloc((void**)&b->h_jobout, (4 * (size_t)njobs + 2) * sizeof(uint64_t), hipHostMallocDefault));
+    HIPCALL(hipMemcpyAsync(b->d_jobs, b->jobs.data(), njobs * sizeof(BatchJob), hipMemcpyHostToDevice, b->stream));
+    HIPCALL(hipMemcpyAsync(b->d_cls_prefix, b->cls_prefix.data(), (njobs + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                           b->stream));
+    HIPCALL(hipMemcpyAsync(b->d_lane_prefix, b->lane_prefix.data(), (njobs + 1) * sizeof(int64_t),
+                           hipMemcpyHostToDevice, b->stream));
+    HIPCALL(hipStreamSynchronize(b->stream));
+    HIPCALL(hipEventCreate(&b->ev0));
+    HIPCALL(hipEventCreate(&b->ev1));
+    return capture(b, &b->g_scan, [&] { return enqueue_scan(b); });
+}
+
+int batch_run(miso_batch* b)
+{
+    HIPCALL(hipEventRecord(b->ev0, b->stream));
+    const uint64_t* tot = b->h_jobout + 4 * b->njobs;
+    if (b->ran && b->g_all) {
+        // capacity known from the last run: all four passes in one graph; the emit pass writes
+        // nothing if the totals outgrew the buffers, and is then run again below
+        HIPCALL(hipGraphLaunch(b->g_all, b->stream));
+        HIPCALL(hipStreamSynchronize(b->stream));
+        if (tot[0] > b->cap_v || tot[1] > b->cap_t) {
+            if (int rc = batch_reserve(b, tot[0], tot[1])) return rc;
+            HIPCALL(hipGraphLaunch(b->g_emit, b->stream));
+        }
+    } else {
+        HIPCALL(hipGraphLaunch(b->g_scan, b->stream));
+        HIPCALL(hipStreamSynchronize(b->stream));
+        if (int rc = batch_reserve(b, tot[0], tot[1])) return rc;
+        if (!b->g_emit)
+            if (int rc = batch_capture_emit(b)) return rc;
+        HIPCALL(hipGraphLaunch(b->g_emit, b->stream));
+    }
+    HIPCALL(hipEventRecord(b->ev1, b->stream));
+    HIPCALL(hipStreamSynchronize(b->stream));
+    HIPCALL(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+    b->ran = true;
+    return MISO_OK;
+}
+
+}  // namespace
+
+extern "C" int miso_batch_create(const miso_volume* volumes, const float* isovals, int64_t njobs, void* stream,
+                                 miso_batch** out)
+{
+    if (!out) return fail(MISO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    miso_batch* b = new (std::nothrow) miso_batch();
+    if (!b) return fail(MISO_ERR_ALLOC, "out of host memory");
+    if (int rc = batch_init(b, volumes, isovals, njobs, stream)) {
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_run(miso_batch* b)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    return batch_run(b);
+}
+
+extern "C" int miso_batch_totals(const miso_batch* b, uint64_t* nverts, uint64_t* ntris)
+{
+    if (!b || !b->ran) return fail(MISO_ERR_STATE, "batch has not run");
+    if (nverts) *nverts = b->h_jobout[4 * b->njobs];
+    if (ntris) *ntris = b->h_jobout[4 * b->njobs + 1];
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_job_counts(const miso_batch* b, int64_t job, uint64_t* nverts, uint64_t* ntris)
+{
+    if (!b || !b->ran) return fail(MISO_ERR_STATE, "batch has not run");
+    if (job < 0 || job >= b->njobs) return fail(MISO_ERR_ARG, "job %lld out of range", (long long)job);
+    if (nverts) *nverts = b->h_jobout[4 * job + 2];
+    if (ntris) *ntris = b->h_jobout[4 * job + 3];
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_job_output(const miso_batch* b, int64_t job, const float** points, const float** normals,
+                                     const uint32_t** triangles)
+{
+    if (!b || !b->ran) return fail(MISO_ERR_STATE, "batch has not run");
+    if (job < 0 || job >= b->njobs) return fail(MISO_ERR_ARG, "job %lld out of range", (long long)job);
+    const uint64_t v0 = b->h_jobout[4 * job], t0 = b->h_jobout[4 * job + 1];
+    if (points) *points = b->d_pts + 3 * v0;
+    if (normals) *normals = b->d_nrm + 3 * v0;
+    if (triangles) *triangles = b->d_tris + 3 * t0;
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_copy_out(miso_batch* b, int64_t job, float* points, float* normals, uint64_t* triangles)
+{
+    if (!b || !b->ran) return fail(MISO_ERR_STATE, "batch has not run");
+    if (job < 0 || job >= b->njobs) return fail(MISO_ERR_ARG, "job %lld out of range", (long long)job);
+    const uint64_t v0 = b->h_jobout[4 * job], t0 = b->h_jobout[4 * job + 1];
+    const uint64_t nv = b->h_jobout[4 * job + 2], nt = b->h_jobout[4 * job + 3];
+    if (points && nv)
+        HIPCALL(hipMemcpyAsync(points, b->d_pts + 3 * v0, 3 * nv * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    if (normals && nv)
+        HIPCALL(hipMemcpyAsync(normals, b->d_nrm + 3 * v0, 3 * nv * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    if (triangles && nt) {
+        uint32_t* tmp = reinterpret_cast<uint32_t*>(triangles) + 3 * nt;
+        HIPCALL(hipMemcpyAsync(tmp, b->d_tris + 3 * t0, 3 * nt * sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+        HIPCALL(hipStreamSynchronize(b->stream));
+        for (uint64_t q = 0; q < 3 * nt; ++q) triangles[q] = tmp[q];
+    }
+    HIPCALL(hipStreamSynchronize(b->stream));
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_set_tuning(miso_batch* b, int blocks_per_cu, int chunk)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    if (blocks_per_cu < 1 || blocks_per_cu > 64 || chunk < 1 || chunk > 1024)
+        return fail(MISO_ERR_ARG, "tuning out of range (blocks_per_cu 1..64, chunk 1..1024)");
+    if (blocks_per_cu == b->blocks_per_cu && chunk == b->chunk) return MISO_OK;
+    HIPCALL(hipStreamSynchronize(b->stream));
+    b->blocks_per_cu = blocks_per_cu;
+    b->chunk = chunk;
+    destroy_graphs(b);
+    if (int rc = capture(b, &b->g_scan, [&] { return enqueue_scan(b); })) return rc;
+    if (b->d_pts) return batch_capture_emit(b);
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_get_tuning(const miso_batch* b, int* blocks_per_cu, int* chunk)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+    if (chunk) *chunk = b->chunk;
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_autotune(miso_batch* b, int reps, float* best_ms)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    if (reps < 1) reps = 1;
+    static const int kBlocks[] = {2, 4, 8};
+    static const int kChunk[] = {1, 4, 16};
+    float best = -1.0f;
+    int bb = b->blocks_per_cu, bc = b->chunk;
+    for (int x : kBlocks)
+        for (int c : kChunk) {
+            if (int rc = miso_batch_set_tuning(b, x, c)) return rc;
+            if (int rc = batch_run(b)) return rc;   // warm: sizes the output, captures the graphs
+            float sum = 0.0f;
+            for (int r = 0; r < reps; ++r) {
+                if (int rc = batch_run(b)) return rc;
+                sum += b->last_ms;
+            }
+            const float ms = sum / reps;
+            if (best < 0.0f || ms < best) {
+                best = ms;
+                bb = x;
+                bc = c;
+            }
+        }
+    if (int rc = miso_batch_set_tuning(b, bb, bc)) return rc;
+    if (best_ms) *best_ms = best;
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_time_ms(const miso_batch* b, float* ms)
+{
+    if (!b || !b->ran) return fail(MISO_ERR_STATE, "batch has not run");
+    if (ms) *ms = b->last_ms;
+    return MISO_OK;
+}
+
+extern "C" void miso_batch_destroy(miso_batch* b)
+{
+    if (!b) return;
+    batch_release(b);
+    delete b;
+}

@@ -9,6 +9,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,7 @@
 #include "fe_kernels.h"
 #include "mc_kernels.h"
 #include "fe_rows.h"
+#include "fe_batch.h"
 
 using namespace miso;
 
@@ -743,6 +745,8 @@ extern "C" void miso_mc_destroy(miso_mc* h)
 }
 
 /* ====================================================================== library */
+
+#include "batch_host.h"
 
 extern "C" const char* miso_last_error(void) { return g_err.c_str(); }
 

@@ -1,0 +1,154 @@
+/*
+ * fe_batch.h -- Flying Edges over many volumes (or one volume at many isovalues) in one launch
+ * per pass.
+ *
+ * The jobs' grid rows are concatenated into one row space: the per-row arrays (sign masks,
+ * counts, trims, offsets) of job j start at row_base / mask_base. Each pass is one persistent
+ * kernel whose blocks take chunks of work items from a device ticket counter; an item is one
+ * block of the single-volume kernels (4 grid rows for classification, 4 wavefronts of lane rows
+ * for counting and emission) of one job, found by a binary search of the jobs' item prefix.
+ * Pass 3 is the single-volume scan over the concatenated rows, so job j's output is the
+ * contiguous range [voff[row_base_j], voff[row_base_{j+1}]) of one shared buffer; its triangles
+ * index its own vertices from 0.
+ */
+#ifndef MINIISO_FE_BATCH_H
+#define MINIISO_FE_BATCH_H
+
+#include "fe_rows.h"
+
+namespace miso {
+
+struct BatchJob {
+    Geom g;
+    const float* vol;
+    float iso;
+    int unit;            // unit spacing and zero origin (the exact fast path of pass 4)
+    int64_t row_base;    // first row of the job in the concatenated row arrays
+    int64_t mask_base;   // first sign-mask word
+};
+
+/* the job whose item range holds `item`: the largest j < njobs with prefix[j] <= item */
+__device__ __forceinline__ int find_job(const int64_t* __restrict__ prefix, int njobs, int64_t item)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+/* Persistent loop: the block takes `chunk` consecutive items per ticket until the queue is
+ * drained (every block exits once a ticket lands past nitems); body(job, item - prefix[job]). */
+template <class Body>
+__device__ __forceinline__ void run_queue(uint64_t* __restrict__ ticket, const int64_t* __restrict__ prefix,
+                                          int njobs, int64_t nitems, int chunk, Body body)
+{
+    __shared__ int64_t s_base;
+    __shared__ int s_job;
+    for (;;) {
+        __syncthreads();   // every thread has read the previous ticket
+        if (threadIdx.x == 0) {
+            const int64_t b = (int64_t)__hip_atomic_fetch_add((unsigned long long*)ticket, (unsigned long long)chunk,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_base = b;
+            s_job = b < nitems ? find_job(prefix, njobs, b) : 0;
+        }
+        __syncthreads();
+        const int64_t base = s_base;
+        if (base >= nitems) break;
+        int j = __builtin_amdgcn_readfirstlane(s_job);
+        const int64_t end = base + chunk < nitems ? base + chunk : nitems;
+        for (int64_t it = base; it < end; ++it) {
+            while (it >= prefix[j + 1]) ++j;
+            body(j, it - prefix[j]);
+        }
+    }
+}
+
+/* clears the queue tickets and the scan's tile status (a kernel, so it can sit in a graph) */
+__global__ __launch_bounds__(256) void k_batch_zero(uint64_t* __restrict__ words, int64_t n)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        words[t] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_batch_classify(const BatchJob* __restrict__ jobs,
+                                                        const int64_t* __restrict__ prefix, int njobs,
+                                                        int64_t nitems, int chunk, uint64_t* __restrict__ mask,
+                                                        uint64_t* __restrict__ ticket)
+{
+    run_queue(ticket, prefix, njobs, nitems, chunk, [&](int j, int64_t blk) {
+        const BatchJob& J = jobs[j];
+        fe_classify_block(J.vol, J.g, J.iso, mask + J.mask_base, blk);
+    });
+}
+
+__global__ __launch_bounds__(256) void k_batch_count(const BatchJob* __restrict__ jobs,
+                                                     const int64_t* __restrict__ prefix, int njobs, int64_t nitems,
+                                                     int chunk, const uint64_t* __restrict__ mask,
+                                                     RowCounts* __restrict__ counts, RowTrim* __restrict__ trims,
+                                                     uint64_t* __restrict__ ticket)
+{
+    run_queue(ticket, prefix, njobs, nitems, chunk, [&](int j, int64_t blk) {
+        const BatchJob& J = jobs[j];
+        fe_count_rows_block(mask + J.mask_base, J.g, counts + J.row_base, trims + J.row_base, blk);
+    });
+}
+
+/* Per job: first vertex, first triangle, vertex and triangle counts (from the scan). */
+__global__ __launch_bounds__(256) void k_batch_job_counts(const BatchJob* __restrict__ jobs, int njobs, int64_t R,
+                                                          const uint64_t* __restrict__ voff,
+                                                          const uint64_t* __restrict__ toff,
+                                                          const uint64_t* __restrict__ totals,
+                                                          uint64_t* __restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const int64_t b = jobs[j].row_base;
+    const int64_t e = j + 1 < njobs ? jobs[j + 1].row_base : R;
+    const uint64_t vb = b < R ? voff[b] : totals[0], tb = b < R ? toff[b] : totals[1];
+    const uint64_t ve = e < R ? voff[e] : totals[0], te = e < R ? toff[e] : totals[1];
+    out[4 * j + 0] = vb;
+    out[4 * j + 1] = tb;
+    out[4 * j + 2] = ve - vb;
+    out[4 * j + 3] = te - tb;
+}
+
+/* Pass 4 for all jobs; writes nothing when the totals exceed the output capacity (the host then
+ * grows the buffers and runs this pass again). */
+__global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__ jobs,
+                                                    const int64_t* __restrict__ prefix, int njobs, int64_t nitems,
+                                                    int chunk, const uint64_t* __restrict__ mask,
+                                                    const RowCounts* __restrict__ counts,
+                                                    const uint64_t* __restrict__ voff,
+                                                    const uint64_t* __restrict__ toff,
+                                                    const uint64_t* __restrict__ job_first,
+                                                    const uint64_t* __restrict__ totals, uint64_t cap_v,
+                                                    uint64_t cap_t, float* __restrict__ pts, float* __restrict__ nrm,
+                                                    uint32_t* __restrict__ tris, uint64_t* __restrict__ ticket)
+{
+    __shared__ uint64_t s_case[256];
+    __shared__ EmitRowsLds s_w[4];
+    if (totals[0] > cap_v || totals[1] > cap_t) return;
+    s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
+    run_queue(ticket, prefix, njobs, nitems, chunk, [&](int j, int64_t blk) {
+        const BatchJob& J = jobs[j];
+        const uint64_t vbias = job_first[4 * j];
+        if (J.unit)
+            fe_emit_rows_block<uint32_t, true>(s_case, s_w, J.vol, mask + J.mask_base, J.g, J.iso,
+                                               counts + J.row_base, voff + J.row_base, toff + J.row_base, pts, nrm,
+                                               tris, vbias, blk);
+        else
+            fe_emit_rows_block<uint32_t, false>(s_case, s_w, J.vol, mask + J.mask_base, J.g, J.iso,
+                                                counts + J.row_base, voff + J.row_base, toff + J.row_base, pts, nrm,
+                                                tris, vbias, blk);
+    });
+}
+
+}  // namespace miso
+
+#endif

@@ -29,11 +29,12 @@ constexpr uint64_t kFlagPre = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
 
 /* ------------------------------------------------------------------ pass 1a: sign mask */
-__global__ __launch_bounds__(256) void k_fe_classify(const float* __restrict__ vol, Geom g, float iso,
-                                                     uint64_t* __restrict__ mask)
+/* rows 4 blk .. 4 blk + 3, one per wavefront */
+__device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol, const Geom& g, float iso,
+                                                  uint64_t* __restrict__ mask, int64_t blk)
 {
     const int lane = lane_id();
-    const int64_t r = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x >> 6);
+    const int64_t r = blk * kRowsPerBlock + (threadIdx.x >> 6);
     if (r >= g.ny * g.nz) return;
     const float* __restrict__ row = vol + r * g.nx;
     uint64_t* __restrict__ mrow = mask + r * g.W;
@@ -51,6 +52,12 @@ __global__ __launch_bounds__(256) void k_fe_classify(const float* __restrict__ v
             if (lane == 0 && w0 + u < g.W) mrow[w0 + u] = b;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_fe_classify(const float* __restrict__ vol, Geom g, float iso,
+                                                     uint64_t* __restrict__ mask)
+{
+    fe_classify_block(vol, g, iso, mask, blockIdx.x);
 }
 
 /* ------------------------------------------------------------------ pass 1b + 2: counts */

@@ -186,18 +186,13 @@ __device__ __forceinline__ uint32_t word_tri_count(const CornerWords& C, uint64_
 }
 
 /* ------------------------------------------------------------------ pass 1b + 2 */
-__global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restrict__ mask, Geom g,
-                                                       RowCounts* __restrict__ counts,
-                                                       RowTrim* __restrict__ trims,
-                                                       uint64_t* __restrict__ scan_state,
-                                                       int64_t scan_state_words)
+/* the 4 wavefronts of lane-row block blk (rows 4 rpw blk ..) */
+__device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__ mask, const Geom& g,
+                                                    RowCounts* __restrict__ counts, RowTrim* __restrict__ trims,
+                                                    int64_t blk)
 {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < scan_state_words;
-         t += (int64_t)gridDim.x * blockDim.x)
-        scan_state[t] = 0;
-
     const RowLanes rl = row_lanes(g.W);
-    const int64_t wave_id = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t wave_id = blk * 4 + (threadIdx.x >> 6);
     const int64_t r = wave_id * rl.rpw + rl.q;
     const int64_t R = g.ny * g.nz;
     const bool ok = r < R && rl.w < g.W;
@@ -223,6 +218,19 @@ __global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restric
         counts[r] = RowCounts{xc, yc, zc, tc};
         trims[r] = RowTrim{lastp1 == 0 ? (uint32_t)g.nx : first, lastp1};
     }
+}
+
+__global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restrict__ mask, Geom g,
+                                                       RowCounts* __restrict__ counts,
+                                                       RowTrim* __restrict__ trims,
+                                                       uint64_t* __restrict__ scan_state,
+                                                       int64_t scan_state_words)
+{
+    // the scan of pass 3 runs next on the same stream; clear its tickets and tile status here
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < scan_state_words;
+         t += (int64_t)gridDim.x * blockDim.x)
+        scan_state[t] = 0;
+    fe_count_rows_block(mask, g, counts, trims, blockIdx.x);
 }
 
 /* ------------------------------------------------------------------ pass 4: edge points */
@@ -440,26 +448,23 @@ __device__ __forceinline__ uint32_t case_of_half(const uint32_t h[8], int b)
  * rows (one vertex per lane from an LDS list in output order, two per lane per step so that the
  * gathers of both are in flight together), then the triangles of its cell rows (active cells
  * listed per lane, resolved one cell and then one triangle per lane). */
+/* Lane-row block blk. s_case holds kCaseEdges; vertex ids are written less vbias (the first
+ * vertex of this volume's output when several volumes share one buffer). */
 template <typename Index, bool Unit>
-__global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ vol,
-                                                      const uint64_t* __restrict__ mask, Geom g, float iso,
-                                                      const RowCounts* __restrict__ counts,
-                                                      const uint64_t* __restrict__ voff,
-                                                      const uint64_t* __restrict__ toff,
-                                                      float* __restrict__ pts, float* __restrict__ nrm,
-                                                      Index* __restrict__ tris)
+__device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ s_case, EmitRowsLds* s_w,
+                                                   const float* __restrict__ vol, const uint64_t* __restrict__ mask,
+                                                   const Geom& g, float iso, const RowCounts* __restrict__ counts,
+                                                   const uint64_t* __restrict__ voff,
+                                                   const uint64_t* __restrict__ toff, float* __restrict__ pts,
+                                                   float* __restrict__ nrm, Index* __restrict__ tris,
+                                                   uint64_t vbias, int64_t blk)
 {
-    __shared__ uint64_t s_case[256];
-    __shared__ EmitRowsLds s_w[4];
-    const int t = threadIdx.x;
     const int lane = lane_id();
-    const int wv = t >> 6;
-    s_case[t] = kCaseEdges[t];
-    __syncthreads();
+    const int wv = threadIdx.x >> 6;
     EmitRowsLds& S = s_w[wv];
     const RowLanes rl = row_lanes(g.W);
     const int64_t R = g.ny * g.nz;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * rl.rpw;
+    const int64_t r0 = (blk * 4 + wv) * rl.rpw;
     if (r0 >= R) return;
     const int64_t r = r0 + rl.q;
     const bool ok = r < R && rl.w < g.W;
@@ -588,6 +593,7 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
     const uint32_t clo[8] = {lo[0], lo[1], lo[3], lo[2], lo[4], lo[5], lo[7], lo[6]};
     const uint32_t chi[8] = {hi[0], hi[1], hi[3], hi[2], hi[4], hi[5], hi[7], hi[6]};
     const uint32_t cfirst = cincl - ncell;
+    const uint64_t vrel = v0 - vbias;
     uint64_t slot = t0;
     for (uint32_t win = 0; win < ctotal; win += kEmitCellCap) {
         const uint32_t wend = win + kEmitCellCap;
@@ -633,12 +639,29 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
                     const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
                     id[s] = S.base[q][owner] + (uint32_t)__popcll(below);
                 }
-                store_tri<Index>(tris, slot + e, v0 + id[0], v0 + id[1], v0 + id[2]);
+                store_tri<Index>(tris, slot + e, vrel + id[0], vrel + id[1], vrel + id[2]);
             }
             slot += ttot;
             wave_sync();
         }
     }
+}
+
+template <typename Index, bool Unit>
+__global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ vol,
+                                                      const uint64_t* __restrict__ mask, Geom g, float iso,
+                                                      const RowCounts* __restrict__ counts,
+                                                      const uint64_t* __restrict__ voff,
+                                                      const uint64_t* __restrict__ toff,
+                                                      float* __restrict__ pts, float* __restrict__ nrm,
+                                                      Index* __restrict__ tris)
+{
+    __shared__ uint64_t s_case[256];
+    __shared__ EmitRowsLds s_w[4];
+    s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
+    __syncthreads();
+    fe_emit_rows_block<Index, Unit>(s_case, s_w, vol, mask, g, iso, counts, voff, toff, pts, nrm, tris, 0,
+                                    blockIdx.x);
 }
 
 }  // namespace miso

@@ -1,0 +1,128 @@
+"""GPU: batched Flying Edges (one persistent launch per pass over many jobs, HIP-graph replay).
+
+Bar: every job's points, normals and triangles bit-identical to the single-volume extraction
+of the same volume and isovalue, whose parity with the reference the single-volume tests pin
+(and, for a few jobs, directly to the oracle); the same on every graph replay and after the
+launch shape changes."""
+import numpy as np
+import pytest
+
+from helpers import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(seed=3):
+    import torch
+    from miniisosurface_amd import DeviceVolume, synth
+    rng = np.random.default_rng(seed)
+    vols = []
+    for dims in [(32, 32, 32), (45, 77, 33), (256, 40, 37), (129, 65, 2), (2, 3, 5), (200, 31, 64), (64, 64, 64),
+                 (97, 130, 41), (33, 250, 38)]:
+        vols.append(synth.gaussians(dims, count=int(rng.integers(1, 7)), seed=int(rng.integers(1 << 30)),
+                                    sigma_range=(4.0, 12.0)))
+    vols.append(synth.gyroid(48, 5))
+    tens = [torch.from_numpy(v).cuda() for v in vols]
+    jobs = []
+    for q, t in enumerate(tens):
+        iso = 0.0 if q == len(tens) - 1 else float(rng.uniform(0.2, 0.8))
+        jobs.append((DeviceVolume.from_tensor(t), iso, vols[q], (1, 1, 1), (0, 0, 0)))
+    # one volume at several isovalues, and one with spacing and origin
+    for iso in (-0.6, 0.25, 0.9):
+        jobs.append((DeviceVolume.from_tensor(tens[-1]), iso, vols[-1], (1, 1, 1), (0, 0, 0)))
+    sp, org = (0.7, 1.3, 0.45), (-3.1, 2.2, 10.5)
+    jobs.append((DeviceVolume.from_tensor(tens[1], sp, org), 0.4, vols[1], sp, org))
+    return jobs, tens
+
+
+def _single(vol, iso, sp, org):
+    import miniisosurface_amd as m
+    a = m.FlyingEdgesAlgorithm(m.Image3D(vol, sp, org), iso)
+    a.run()
+    return a.moveOutput()
+
+
+def _check(b, refs):
+    for q, ref in enumerate(refs):
+        got = b.mesh(q)
+        assert got.triangles.shape == ref.triangles.shape, q
+        assert bits_equal(got.triangles, ref.triangles), q
+        assert bits_equal(got.points, ref.points), q
+        assert bits_equal(got.normals, ref.normals), q
+
+
+def test_batch_matches_single_volume():
+    from miniisosurface_amd.batch import BatchExtraction
+    jobs, tens = _jobs()
+    print(f"[batch] {len(jobs)} jobs", flush=True)
+    refs = [_single(v, iso, sp, org) for _, iso, v, sp, org in jobs]
+    b = BatchExtraction([(dv, iso) for dv, iso, _, _, _ in jobs])
+    for rep in range(3):          # first run (two graphs), then the single-graph replays
+        b.run()
+        _check(b, refs)
+    tv, tt = b.totals()
+    assert tv == sum(r.points.shape[0] for r in refs)
+    assert tt == sum(r.triangles.shape[0] for r in refs)
+    for shape in [(1, 1), (8, 16), (2, 4)]:
+        b.set_tuning(*shape)
+        b.run()
+        _check(b, refs)
+    assert b.time_ms() > 0.0
+    b.close()
+
+
+def test_batch_against_oracle(oracle_mod):
+    from miniisosurface_amd.batch import BatchExtraction
+    jobs, tens = _jobs(seed=11)
+    pick = [0, 2, len(jobs) - 1]
+    b = BatchExtraction([(jobs[q][0], jobs[q][1]) for q in pick]).run()
+    for n, q in enumerate(pick):
+        _, iso, vol, sp, org = jobs[q]
+        r = oracle_mod.FEResult(vol, iso, sp, org, want_debug=False)
+        got = b.mesh(n)
+        assert bits_equal(got.points, r.points), q
+        assert bits_equal(got.normals, r.normals), q
+        assert bits_equal(got.triangles, r.triangles), q
+
+
+def test_batch_autotune_keeps_results():
+    from miniisosurface_amd.batch import BatchExtraction
+    jobs, tens = _jobs(seed=5)
+    refs = [_single(v, iso, sp, org) for _, iso, v, sp, org in jobs]
+    b = BatchExtraction([(dv, iso) for dv, iso, _, _, _ in jobs])
+    best = b.autotune(reps=1)
+    assert best > 0.0
+    assert b.tuning() in [(x, c) for x in (2, 4, 8) for c in (1, 4, 16)]
+    b.run()
+    _check(b, refs)
+
+
+def test_batch_volume_content_change_regrows_output():
+    """Replays from the single graph must notice when the totals outgrow the output."""
+    import torch
+    from miniisosurface_amd import DeviceVolume
+    from miniisosurface_amd.batch import BatchExtraction
+    t = torch.zeros((40, 50, 60), dtype=torch.float32, device="cuda")
+    t[10:20, 10:20, 10:20] = 1.0
+    b = BatchExtraction([(DeviceVolume.from_tensor(t), 0.5)])
+    b.run()
+    v1, t1 = b.totals()
+    b.run()
+    t.copy_(torch.from_numpy(np.random.default_rng(0).uniform(0, 1, (40, 50, 60)).astype(np.float32)))
+    b.run()
+    v2, t2 = b.totals()
+    assert v2 > v1 and t2 > t1
+    ref = _single(t.cpu().numpy(), 0.5, (1, 1, 1), (0, 0, 0))
+    _check(b, [ref])
+
+
+def test_batch_rejects_host_and_flat_volumes():
+    import miniisosurface_amd as m
+    import torch
+    from miniisosurface_amd import DeviceVolume
+    from miniisosurface_amd.batch import BatchExtraction
+    with pytest.raises(ValueError):
+        BatchExtraction([(m.Image3D(np.zeros((4, 4, 4), np.float32)), 0.5)])
+    t = torch.zeros((1, 4, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(m.MisoError):
+        BatchExtraction([(DeviceVolume.from_tensor(t), 0.5)])
