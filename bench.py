@@ -113,6 +113,80 @@ def read_traffic(kernel):
     return None
 
 
+def bench_batched(reps):
+    """256 mixed-size Gaussian volumes (dims log-uniform in [32, 256], 1-6 blobs, iso U[0.2, 0.8],
+    seed 0), and one 512^3 gyroid at 16 isovalues in [-1, 1]: the batch (one persistent launch per
+    pass, graph replay, autotuned shape) against the same jobs run one extraction at a time."""
+    import numpy as np
+    import torch
+
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+    from miniisosurface_amd.batch import BatchExtraction
+
+    def compare(plain_meshes, b):
+        for q, ref in enumerate(plain_meshes):
+            got = b.mesh(q)
+            if not (np.array_equal(got.triangles, ref.triangles)
+                    and np.array_equal(got.points.view(np.uint32), ref.points.view(np.uint32))
+                    and np.array_equal(got.normals.view(np.uint32), ref.normals.view(np.uint32))):
+                return False
+        return True
+
+    def run_set(name, jobs, algos, set_iso=None):
+        # one extraction at a time: each runs its four passes with its own launches and sync
+        for a in algos[:2]:
+            a.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for q, a in enumerate(algos):
+                if set_iso is not None:
+                    a.set_isoval(set_iso[q])
+                a.run()
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t0) / reps
+        plain_meshes = []
+        for q, a in enumerate(algos):
+            if set_iso is not None:
+                a.set_isoval(set_iso[q])
+                a.run()
+            plain_meshes.append(a.moveOutput())
+        b = BatchExtraction(jobs)
+        tune_ms = b.autotune(reps=2)
+        b.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run()
+        torch.cuda.synchronize()
+        batch_s = (time.perf_counter() - t0) / reps
+        V, T = b.totals()
+        same = compare(plain_meshes, b)
+        out = {"jobs": len(jobs), "plain_ms": round(plain_s * 1e3, 3), "batched_ms": round(batch_s * 1e3, 3),
+               "batched_device_ms": round(b.time_ms(), 3), "speedup": round(plain_s / batch_s, 2),
+               "plain_jobs_per_s": round(len(jobs) / plain_s, 1), "batched_jobs_per_s": round(len(jobs) / batch_s, 1),
+               "nverts": V, "ntris": T, "tuning": list(b.tuning()), "autotune_best_ms": round(tune_ms, 3),
+               "identical_output": same}
+        b.close()
+        return out
+
+    vols = synth.batch_volumes(256, seed=0, device="cuda")
+    dvs = [m.DeviceVolume.from_tensor(t) for t, _ in vols]
+    jobs = [(dv, iso) for dv, (_, iso) in zip(dvs, vols)]
+    algos = [m.FlyingEdgesAlgorithm(dv, iso) for dv, iso in jobs]
+    voxels = sum(int(t.numel()) for t, _ in vols)
+    mixed = run_set("mixed", jobs, algos)
+    mixed["voxels"] = voxels
+    del algos
+    g = synth.gyroid(512, seed=0, device="cuda")
+    dv = m.DeviceVolume.from_tensor(g)
+    isos = [float(x) for x in np.linspace(-1.0, 1.0, 16)]
+    one = m.FlyingEdgesAlgorithm(dv, isos[0])
+    sweep = run_set("sweep", [(dv, i) for i in isos], [one] * 16, set_iso=isos)
+    return {"mixed_256": mixed, "gyroid512_16_isovalues": sweep}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +196,7 @@ def main():
     ap.add_argument("--iso", type=float, default=0.5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 1 GiB device copy first (calibrates rocprofv3 byte counters)")
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT"),
@@ -237,6 +312,8 @@ def main():
         "sparse_iso_0.95": sparse,
         "volume_gen_s": round(gen_s, 2),
     }
+    if not args.no_batch:
+        out["batched"] = bench_batched(reps=3)
     if not args.no_cpu:
         slab = vol[: max(2, n // 4)].cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(slab, args.iso, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))

@@ -32,7 +32,10 @@ def _dims3(n):
     return tuple(int(v) for v in n)
 
 
-def gaussians(n=128, count=4, seed=0, sigma_range=(8.0, 20.0)) -> np.ndarray:
+def gaussians(n=128, count=4, seed=0, sigma_range=(8.0, 20.0), device=None):
+    """device="cuda" evaluates the same float64 formula with torch on the GPU and returns a
+    float32 tensor there (the GPU's exp may differ from the host's in the last float64 bit, so
+    a rare voxel can round to a neighbouring float32)."""
     nx, ny, nz = _dims3(n)
     rng = np.random.default_rng(seed)
     blobs = []
@@ -40,6 +43,17 @@ def gaussians(n=128, count=4, seed=0, sigma_range=(8.0, 20.0)) -> np.ndarray:
         c = rng.uniform(0, [nx - 1, ny - 1, nz - 1], size=3)
         s = rng.uniform(*sigma_range)
         blobs.append((c, s))
+    if device is not None:
+        import torch
+        x = torch.arange(nx, dtype=torch.float64, device=device)
+        y = torch.arange(ny, dtype=torch.float64, device=device)
+        z = torch.arange(nz, dtype=torch.float64, device=device)
+        f = torch.zeros((nz, ny, nx), dtype=torch.float64, device=device)
+        for c, s in blobs:
+            d2 = ((x - c[0]) ** 2)[None, None, :] + ((y - c[1]) ** 2)[None, :, None] \
+                + ((z - c[2]) ** 2)[:, None, None]
+            f += torch.exp(-d2 / (2 * s ** 2))
+        return f.to(torch.float32)
     x = np.arange(nx, dtype=np.float64)
     y = np.arange(ny, dtype=np.float64)
     z = np.arange(nz, dtype=np.float64)
@@ -51,7 +65,8 @@ def gaussians(n=128, count=4, seed=0, sigma_range=(8.0, 20.0)) -> np.ndarray:
     return f.astype(np.float32)
 
 
-def gyroid(n=512, seed=0, periods=8, noise=0.05) -> np.ndarray:
+def gyroid(n=512, seed=0, periods=8, noise=0.05, device=None):
+    """device="cuda": the trigonometric part in float64 on the GPU, the same host noise draws."""
     nx, ny, nz = _dims3(n)
     k = 2.0 * math.pi * periods / nx
     x = np.arange(nx, dtype=np.float64) * k
@@ -60,8 +75,19 @@ def gyroid(n=512, seed=0, periods=8, noise=0.05) -> np.ndarray:
     sx, cx = np.sin(x), np.cos(x)
     sy, cy = np.sin(y), np.cos(y)
     sz, cz = np.sin(z), np.cos(z)
-    out = np.empty((nz, ny, nx), np.float32)
     rng = np.random.default_rng(seed)
+    if device is not None:
+        import torch
+        T = lambda a: torch.from_numpy(a).to(device)
+        tsx, tcy, tsy, tcx = T(sx), T(cy), T(sy), T(cx)
+        out = torch.empty((nz, ny, nx), dtype=torch.float32, device=device)
+        for kk in range(nz):
+            f = tsx[None, :] * tcy[:, None] + tsy[:, None] * float(cz[kk]) + float(sz[kk]) * tcx[None, :]
+            if noise:
+                f = f + T(rng.uniform(-noise, noise, size=(ny, nx)))
+            out[kk] = f.to(torch.float32)
+        return out
+    out = np.empty((nz, ny, nx), np.float32)
     for kk in range(nz):  # plane by plane keeps float64 temporaries small
         f = sx[None, :] * cy[:, None] + sy[:, None] * cz[kk] + sz[kk] * cx[None, :]
         if noise:
@@ -120,7 +146,7 @@ def value_noise(n=1024, seed=0, octaves=3, base_cells=16, device=None):
     return out
 
 
-def batch_volumes(count=256, seed=0, lo=32, hi=256):
+def batch_volumes(count=256, seed=0, lo=32, hi=256, device=None):
     """Mixed-size batch: list of (volume, isovalue)."""
     rng = np.random.default_rng(seed)
     out = []
@@ -132,6 +158,6 @@ def batch_volumes(count=256, seed=0, lo=32, hi=256):
         iso = float(rng.uniform(0.2, 0.8))
         scale = float(min(dims)) / 128.0
         vol = gaussians(tuple(int(d) for d in dims), count=nb, seed=sub_seed,
-                        sigma_range=(8.0 * scale, 20.0 * scale))
+                        sigma_range=(8.0 * scale, 20.0 * scale), device=device)
         out.append((vol, iso))
     return out
