@@ -138,6 +138,22 @@ int miso_batch_autotune(miso_batch* b, int reps, float* best_ms);
 int miso_batch_time_ms(const miso_batch* b, float* ms);
 void miso_batch_destroy(miso_batch* b);
 
+/* Split-phase use for a serving loop (no counterpart in the reference either): a batch without
+ * jobs whose job list is replaced per step, passes 1-3 enqueued without waiting, the counts then
+ * used to admit a prefix of the jobs by the memory their output needs, and pass 4 of that prefix
+ * written into caller-owned device buffers (job j's output starts at its first vertex / first
+ * triangle as given by miso_batch_job_offsets; jobs after the prefix get no output). */
+int miso_batch_create_empty(void* stream, miso_batch** out);
+int miso_batch_set_jobs(miso_batch* b, const miso_volume* volumes, const float* isovals, int64_t njobs);
+int miso_batch_count_async(miso_batch* b);
+/* 1 when the batch's stream is idle, 0 while work is in flight */
+int miso_batch_query(miso_batch* b);
+int miso_batch_count_wait(miso_batch* b);
+int miso_batch_job_offsets(const miso_batch* b, int64_t job, uint64_t* first_vertex, uint64_t* first_triangle);
+int miso_batch_emit_async(miso_batch* b, int64_t njobs_emit, float* points, float* normals, uint32_t* triangles,
+                          uint64_t cap_vertices, uint64_t cap_triangles);
+int miso_batch_wait(miso_batch* b);
+
 /* ---------------------------------------------------------------- library */
 const char* miso_last_error(void);
 /* "miniiso <version> gfx950" */

@@ -81,6 +81,14 @@ def lib():
         "miso_batch_get_tuning": [P, ctypes.POINTER(I), ctypes.POINTER(I)],
         "miso_batch_autotune": [P, I, ctypes.POINTER(F)],
         "miso_batch_time_ms": [P, ctypes.POINTER(F)],
+        "miso_batch_create_empty": [P, PP],
+        "miso_batch_set_jobs": [P, ctypes.POINTER(Volume), ctypes.POINTER(F), ctypes.c_int64],
+        "miso_batch_count_async": [P],
+        "miso_batch_query": [P],
+        "miso_batch_count_wait": [P],
+        "miso_batch_job_offsets": [P, ctypes.c_int64, U64P, U64P],
+        "miso_batch_emit_async": [P, ctypes.c_int64, P, P, P, ctypes.c_uint64, ctypes.c_uint64],
+        "miso_batch_wait": [P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -32,6 +32,9 @@ struct miso_batch {
     uint32_t* d_tris = nullptr;
     uint64_t cap_v = 0, cap_t = 0;
     int blocks_per_cu = 4, chunk = 4, num_cus = 256;
+    int64_t cap_jobs = 0, cap_prefix_c = 0, cap_prefix_l = 0, cap_mask = 0, cap_counts = 0, cap_trims = 0;
+    int64_t cap_voff = 0, cap_toff = 0, cap_state = 0, cap_jobout = 0, cap_host = 0;
+    bool counted = false;             // split-phase use: counts of the current job set are on the host
     hipGraphExec_t g_scan = nullptr, g_emit = nullptr, g_all = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
@@ -102,16 +105,24 @@ int enqueue_scan(miso_batch* b)
     return MISO_OK;
 }
 
-int enqueue_emit(miso_batch* b)
+/* pass 4 for jobs [0, nemit) into the given buffers */
+int enqueue_emit_into(miso_batch* b, int64_t nemit, float* pts, float* nrm, uint32_t* tris, uint64_t cap_v,
+                      uint64_t cap_t)
 {
-    const int64_t nlane = b->lane_prefix.back();
+    const int64_t nlane = b->lane_prefix[nemit];
+    if (nlane == 0) return MISO_OK;
+    const uint64_t* need = nemit < b->njobs ? b->d_jobout + 4 * nemit : b->totals();
     hipLaunchKernelGGL(k_batch_emit, dim3(batch_grid(b, nlane)), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
                        (const int64_t*)b->d_lane_prefix, b->njobs, nlane, b->chunk, (const uint64_t*)b->d_mask,
                        (const RowCounts*)b->d_counts, (const uint64_t*)b->d_voff, (const uint64_t*)b->d_toff,
-                       (const uint64_t*)b->d_jobout, (const uint64_t*)b->totals(), b->cap_v, b->cap_t, b->d_pts,
-                       b->d_nrm, b->d_tris, b->tickets() + 2);
+                       (const uint64_t*)b->d_jobout, need, cap_v, cap_t, pts, nrm, tris, b->tickets() + 2);
     HIPCALL(hipGetLastError());
     return MISO_OK;
+}
+
+int enqueue_emit(miso_batch* b)
+{
+    return enqueue_emit_into(b, b->njobs, b->d_pts, b->d_nrm, b->d_tris, b->cap_v, b->cap_t);
 }
 
 /* captures what `fn` enqueues on the batch stream into an executable graph */
@@ -188,20 +199,47 @@ void batch_release(miso_batch* b)
     b->stream = nullptr;
 }
 
-int batch_init(miso_batch* b, const miso_volume* vols, const float* isovals, int64_t njobs, void* user_stream)
+int batch_setup(miso_batch* b, void* user_stream)
+{
+    if (user_stream) {
+        b->stream = (hipStream_t)user_stream;
+    } else {
+        HIPCALL(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->own_stream = true;
+    }
+    int dev = 0;
+    HIPCALL(hipGetDevice(&dev));
+    HIPCALL(hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCALL(hipEventCreate(&b->ev0));
+    HIPCALL(hipEventCreate(&b->ev1));
+    return MISO_OK;
+}
+
+template <typename T>
+int grow(T** p, int64_t* cap, int64_t need)
+{
+    if (need <= *cap && *p) return MISO_OK;
+    dev_free(*p);
+    *cap = 0;
+    const int64_t n = std::max<int64_t>(need, 1);
+    if (int rc = dev_alloc(p, (size_t)n)) return rc;
+    *cap = n;
+    return MISO_OK;
+}
+
+/* Validates the jobs and lays out their rows; grows the device arrays as needed. */
+int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals, int64_t njobs)
 {
     if (njobs <= 0 || njobs > (1 << 24)) return fail(MISO_ERR_ARG, "job count %lld out of range", (long long)njobs);
     if (!vols || !isovals) return fail(MISO_ERR_ARG, "volumes or isovalues NULL");
-    b->njobs = (int)njobs;
-    b->jobs.resize(njobs);
-    b->cls_prefix.assign(njobs + 1, 0);
-    b->lane_prefix.assign(njobs + 1, 0);
+    std::vector<BatchJob> jobs(njobs);
+    std::vector<int64_t> cls(njobs + 1, 0), lane(njobs + 1, 0);
     int64_t R = 0, mw = 0;
     for (int64_t j = 0; j < njobs; ++j) {
         const miso_volume* v = &vols[j];
         if (int rc = check_volume(v)) return rc;
         if (!v->on_device) return fail(MISO_ERR_ARG, "job %lld: batch volumes must be device-resident", (long long)j);
-        BatchJob& J = b->jobs[j];
+        BatchJob& J = jobs[j];
         J.g = make_geom(v);
         if (!has_cells(J.g)) return fail(MISO_ERR_ARG, "job %lld: dimensions below 2", (long long)j);
         if (!geom_lane_rows(J.g)) return fail(MISO_ERR_ARG, "job %lld: volume too large for a batch", (long long)j);
@@ -216,47 +254,59 @@ int batch_init(miso_batch* b, const miso_volume* vols, const float* isovals, int
         R += rows;
         mw += rows * J.g.W;
         const int64_t per_block = 4 * geom_rows_per_wave(J.g);
-        b->cls_prefix[j + 1] = b->cls_prefix[j] + (rows + kRowsPerBlock - 1) / kRowsPerBlock;
-        b->lane_prefix[j + 1] = b->lane_prefix[j] + (rows + per_block - 1) / per_block;
+        cls[j + 1] = cls[j] + (rows + kRowsPerBlock - 1) / kRowsPerBlock;
+        lane[j + 1] = lane[j] + (rows + per_block - 1) / per_block;
     }
+    HIPCALL(hipStreamSynchronize(b->stream));   // the previous job set is no longer in use
+    destroy_graphs(b);
+    b->ran = false;
+    b->njobs = (int)njobs;
+    b->jobs.swap(jobs);
+    b->cls_prefix.swap(cls);
+    b->lane_prefix.swap(lane);
     b->R = R;
     b->mask_words = mw;
     b->ntiles = (R + kScanTile - 1) / kScanTile;
     b->state_words = 5 + 1 + 2 * b->ntiles;
-    if (user_stream) {
-        b->stream = (hipStream_t)user_stream;
-    } else {
-        HIPCALL(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-        b->own_stream = true;
-    }
-    int dev = 0;
-    HIPCALL(hipGetDevice(&dev));
-    HIPCALL(hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     int rc = 0;
-    if ((rc = dev_alloc(&b->d_jobs, (size_t)njobs))) return rc;
-    if ((rc = dev_alloc(&b->d_cls_prefix, (size_t)njobs + 1))) return rc;
-    if ((rc = dev_alloc(&b->d_lane_prefix, (size_t)njobs + 1))) return rc;
-    if ((rc = dev_alloc(&b->d_mask, (size_t)mw))) return rc;
-    if ((rc = dev_alloc(&b->d_counts, (size_t)R))) return rc;
-    if ((rc = dev_alloc(&b->d_trims, (size_t)R))) return rc;
-    if ((rc = dev_alloc(&b->d_voff, (size_t)R))) return rc;
-    if ((rc = dev_alloc(&b->d_toff, (size_t)R))) return rc;
-    if ((rc = dev_alloc(&b->d_state, (size_t)b->state_words))) return rc;
-    if ((rc = dev_alloc(&b->d_jobout, 4 * (size_t)njobs))) return rc;
-    HIPCALL(hipHostMalloc((void**)&b->h_jobout, (4 * (size_t)njobs + 2) * sizeof(uint64_t), hipHostMallocDefault));
+    if ((rc = grow(&b->d_jobs, &b->cap_jobs, njobs))) return rc;
+    if ((rc = grow(&b->d_cls_prefix, &b->cap_prefix_c, njobs + 1))) return rc;
+    if ((rc = grow(&b->d_lane_prefix, &b->cap_prefix_l, njobs + 1))) return rc;
+    if ((rc = grow(&b->d_mask, &b->cap_mask, mw))) return rc;
+    if ((rc = grow(&b->d_counts, &b->cap_counts, R))) return rc;
+    if ((rc = grow(&b->d_trims, &b->cap_trims, R))) return rc;
+    if ((rc = grow(&b->d_voff, &b->cap_voff, R))) return rc;
+    if ((rc = grow(&b->d_toff, &b->cap_toff, R))) return rc;
+    if ((rc = grow(&b->d_state, &b->cap_state, b->state_words))) return rc;
+    if ((rc = grow(&b->d_jobout, &b->cap_jobout, 4 * njobs))) return rc;
+    if (njobs + 1 > b->cap_host) {
+        if (b->h_jobout) (void)hipHostFree(b->h_jobout);
+        b->h_jobout = nullptr;
+        b->cap_host = 0;
+        HIPCALL(hipHostMalloc((void**)&b->h_jobout, (4 * (size_t)njobs + 2) * sizeof(uint64_t), hipHostMallocDefault));
+        b->cap_host = njobs + 1;
+    }
     HIPCALL(hipMemcpyAsync(b->d_jobs, b->jobs.data(), njobs * sizeof(BatchJob), hipMemcpyHostToDevice, b->stream));
     HIPCALL(hipMemcpyAsync(b->d_cls_prefix, b->cls_prefix.data(), (njobs + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
                            b->stream));
     HIPCALL(hipMemcpyAsync(b->d_lane_prefix, b->lane_prefix.data(), (njobs + 1) * sizeof(int64_t),
                            hipMemcpyHostToDevice, b->stream));
     HIPCALL(hipStreamSynchronize(b->stream));
-    HIPCALL(hipEventCreate(&b->ev0));
-    HIPCALL(hipEventCreate(&b->ev1));
+    return MISO_OK;
+}
+
+int batch_init(miso_batch* b, const miso_volume* vols, const float* isovals, int64_t njobs, void* user_stream)
+{
+    if (int rc = batch_setup(b, user_stream)) return rc;
+    if (int rc = batch_set_jobs(b, vols, isovals, njobs)) return rc;
     return capture(b, &b->g_scan, [&] { return enqueue_scan(b); });
 }
 
 int batch_run(miso_batch* b)
 {
+    if (b->njobs == 0) return fail(MISO_ERR_STATE, "batch has no jobs");
+    if (!b->g_scan)
+        if (int rc = capture(b, &b->g_scan, [&] { return enqueue_scan(b); })) return rc;
     HIPCALL(hipEventRecord(b->ev0, b->stream));
     const uint64_t* tot = b->h_jobout + 4 * b->njobs;
     if (b->ran && b->g_all) {
@@ -280,6 +330,7 @@ int batch_run(miso_batch* b)
     HIPCALL(hipStreamSynchronize(b->stream));
     HIPCALL(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
     b->ran = true;
+    b->counted = true;
     return MISO_OK;
 }
 
@@ -366,6 +417,7 @@ extern "C" int miso_batch_set_tuning(miso_batch* b, int blocks_per_cu, int chunk
     b->blocks_per_cu = blocks_per_cu;
     b->chunk = chunk;
     destroy_graphs(b);
+    if (b->njobs == 0) return MISO_OK;
     if (int rc = capture(b, &b->g_scan, [&] { return enqueue_scan(b); })) return rc;
     if (b->d_pts) return batch_capture_emit(b);
     return MISO_OK;
@@ -405,6 +457,87 @@ extern "C" int miso_batch_autotune(miso_batch* b, int reps, float* best_ms)
         }
     if (int rc = miso_batch_set_tuning(b, bb, bc)) return rc;
     if (best_ms) *best_ms = best;
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_create_empty(void* stream, miso_batch** out)
+{
+    if (!out) return fail(MISO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    miso_batch* b = new (std::nothrow) miso_batch();
+    if (!b) return fail(MISO_ERR_ALLOC, "out of host memory");
+    if (int rc = batch_setup(b, stream)) {
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_set_jobs(miso_batch* b, const miso_volume* volumes, const float* isovals, int64_t njobs)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    b->counted = false;
+    return batch_set_jobs(b, volumes, isovals, njobs);
+}
+
+extern "C" int miso_batch_count_async(miso_batch* b)
+{
+    if (!b || b->njobs == 0) return fail(MISO_ERR_STATE, "batch has no jobs");
+    b->counted = false;
+    HIPCALL(hipEventRecord(b->ev0, b->stream));
+    return enqueue_scan(b);
+}
+
+extern "C" int miso_batch_query(miso_batch* b)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(MISO_ERR_HIP, "hipStreamQuery: %s", hipGetErrorString(e));
+}
+
+extern "C" int miso_batch_count_wait(miso_batch* b)
+{
+    if (!b || b->njobs == 0) return fail(MISO_ERR_STATE, "batch has no jobs");
+    HIPCALL(hipStreamSynchronize(b->stream));
+    b->counted = true;
+    b->ran = true;   // per-job counts and totals are valid
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_job_offsets(const miso_batch* b, int64_t job, uint64_t* first_vertex,
+                                      uint64_t* first_triangle)
+{
+    if (!b || !b->counted) return fail(MISO_ERR_STATE, "batch has not been counted");
+    if (job < 0 || job >= b->njobs) return fail(MISO_ERR_ARG, "job %lld out of range", (long long)job);
+    if (first_vertex) *first_vertex = b->h_jobout[4 * job];
+    if (first_triangle) *first_triangle = b->h_jobout[4 * job + 1];
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_emit_async(miso_batch* b, int64_t njobs_emit, float* points, float* normals,
+                                     uint32_t* triangles, uint64_t cap_vertices, uint64_t cap_triangles)
+{
+    if (!b || !b->counted) return fail(MISO_ERR_STATE, "batch has not been counted");
+    if (njobs_emit < 0 || njobs_emit > b->njobs) return fail(MISO_ERR_ARG, "njobs_emit out of range");
+    const uint64_t nv = njobs_emit < b->njobs ? b->h_jobout[4 * njobs_emit] : b->h_jobout[4 * b->njobs];
+    const uint64_t nt = njobs_emit < b->njobs ? b->h_jobout[4 * njobs_emit + 1] : b->h_jobout[4 * b->njobs + 1];
+    if (nv > cap_vertices || nt > cap_triangles)
+        return fail(MISO_ERR_ARG, "output of %llu vertices / %llu triangles exceeds the buffers", (unsigned long long)nv,
+                    (unsigned long long)nt);
+    if ((nv && (!points || !normals)) || (nt && !triangles)) return fail(MISO_ERR_ARG, "output buffer NULL");
+    if (int rc = enqueue_emit_into(b, njobs_emit, points, normals, triangles, cap_vertices, cap_triangles)) return rc;
+    HIPCALL(hipEventRecord(b->ev1, b->stream));
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_wait(miso_batch* b)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    HIPCALL(hipStreamSynchronize(b->stream));
     return MISO_OK;
 }
 

@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void k_batch_job_counts(const BatchJob* __rest
     out[4 * j + 3] = te - tb;
 }
 
-/* Pass 4 for all jobs; writes nothing when the totals exceed the output capacity (the host then
- * grows the buffers and runs this pass again). */
+/* Pass 4 for the first jobs (items [0, nitems)); writes nothing when their output exceeds the
+ * capacity (the host then grows the buffers and runs this pass again). */
 __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__ jobs,
                                                     const int64_t* __restrict__ prefix, int njobs, int64_t nitems,
                                                     int chunk, const uint64_t* __restrict__ mask,
@@ -127,13 +127,15 @@ __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__
                                                     const uint64_t* __restrict__ voff,
                                                     const uint64_t* __restrict__ toff,
                                                     const uint64_t* __restrict__ job_first,
-                                                    const uint64_t* __restrict__ totals, uint64_t cap_v,
+                                                    const uint64_t* __restrict__ need, uint64_t cap_v,
                                                     uint64_t cap_t, float* __restrict__ pts, float* __restrict__ nrm,
                                                     uint32_t* __restrict__ tris, uint64_t* __restrict__ ticket)
 {
     __shared__ uint64_t s_case[256];
     __shared__ EmitRowsLds s_w[4];
-    if (totals[0] > cap_v || totals[1] > cap_t) return;
+    // need: vertices and triangles of the emitted jobs (the totals, or the first vertex and
+    // triangle of the first job not emitted)
+    if (need[0] > cap_v || need[1] > cap_t) return;
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
     run_queue(ticket, prefix, njobs, nitems, chunk, [&](int j, int64_t blk) {
         const BatchJob& J = jobs[j];
