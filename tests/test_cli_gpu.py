@@ -1,0 +1,114 @@
+"""GPU: the command-line drivers (flags, YAML report, mesh file) against the reference drivers
+compiled into oracle/_ref when present, else against the oracle's mesh written by save_mesh
+(whose bytes equal the reference writer's, test_tools_cpu.py)."""
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from helpers import golden_files, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = ["rand20_iso050", "spacing_origin_24x19x14", "thin_64x3x2", "c1_gaussians_128"]
+
+
+def _tool(name):
+    from miniisosurface_amd import apps
+    p = apps.tool_path(name)
+    assert os.path.exists(p), f"{p} not built (python -m miniisosurface_amd.build; apps.build())"
+    return p
+
+
+def _expected(oracle_mod, tmp, exe_name, vol, iso, sp, org, mc=False):
+    from miniisosurface_amd import save_mesh
+    ref = oracle_mod.ref_binary(exe_name)
+    out = os.path.join(tmp, "expected.vtk")
+    if ref:
+        subprocess.run([ref, "-i", "v.vtk", "-o", "expected.vtk", "-v", repr(float(iso)), "-y", "ry/r.yaml"],
+                       cwd=tmp, check=True, capture_output=True, timeout=300)
+    else:
+        r = oracle_mod.MCResult(vol, iso, sp, org) if mc else oracle_mod.FEResult(vol, iso, sp, org, want_debug=False)
+        from miniisosurface_amd import TriangleMesh
+        save_mesh(out, TriangleMesh(r.points, r.normals, r.triangles))
+    return open(out, "rb").read()
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_flying_edges_cli_matches_reference_driver(oracle_mod, name):
+    from miniisosurface_amd import Image3D, save_image
+    p = [g for g in golden_files() if name in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    print(f"[cli fe] {name}", flush=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        out = subprocess.run([_tool("miniiso_flying_edges"), "-i", "v.vtk", "-o", "ours.vtk", "-v", repr(float(iso)),
+                              "-y", "yy/report.yaml"], cwd=tmp, capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+        ours = open(os.path.join(tmp, "ours.vtk"), "rb").read()
+        assert ours == _expected(oracle_mod, tmp, "flyingEdgesSerial", vol, iso, sp, org)
+        report = open(os.path.join(tmp, "yy", "report.yaml")).read()
+        assert report == out.stdout
+        assert report.startswith("Mini-Application Name: Flying Edges\nMini-Application Version: 0.1\n")
+        for key in ("Number of vertices in mesh", "Number of triangles in mesh", "Pass 1", "  Wall Time (seconds)",
+                    "Total Program WALL Time (seconds)", "File x-dimension"):
+            assert key in report, key
+        nv = [l for l in report.splitlines() if l.startswith("Number of vertices in mesh: ")][0]
+        assert int(nv.split(": ")[1]) == d["fe_points"].shape[0]
+
+
+@pytest.mark.parametrize("name", FIXTURES[:3])
+def test_marching_cubes_cli_matches_reference_driver(oracle_mod, name):
+    from miniisosurface_amd import Image3D, save_image
+    p = [g for g in golden_files() if name in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        out = subprocess.run([_tool("miniiso_marching_cubes"), "-i", "v.vtk", "-o", "ours.vtk", "-v",
+                              repr(float(iso)), "-y", "yy/report.yaml"], cwd=tmp, capture_output=True, text=True,
+                             timeout=300)
+        assert out.returncode == 0, out.stderr
+        ours = open(os.path.join(tmp, "ours.vtk"), "rb").read()
+        assert ours == _expected(oracle_mod, tmp, "mcSerial", vol, iso, sp, org, mc=True)
+        assert "Mini-Application Name: Marching Cubes" in out.stdout
+
+
+def test_marching_cubes_cli_dat_input(oracle_mod):
+    """-input_dat: 16-bit little-endian dims and samples, unit spacing, zero origin."""
+    from miniisosurface_amd import TriangleMesh, load_mesh
+    rng = np.random.default_rng(3)
+    vol = rng.integers(0, 1000, (9, 11, 13)).astype(np.uint16)
+    with tempfile.TemporaryDirectory() as tmp:
+        with open(os.path.join(tmp, "v.dat"), "wb") as f:
+            f.write(struct.pack("<3H", 13, 11, 9))
+            f.write(vol.astype("<u2").tobytes())
+        out = subprocess.run([_tool("miniiso_marching_cubes"), "-input_dat", "v.dat", "-o", "ours.vtk", "-v", "500",
+                              "-y", "yy/r.yaml"], cwd=tmp, capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+        m = load_mesh(os.path.join(tmp, "ours.vtk"))
+    r = oracle_mod.MCResult(vol.astype(np.float32), 500.0)
+    assert np.array_equal(m.triangles, r.triangles)
+    assert np.array_equal(m.points.view(np.uint32), r.points.view(np.uint32))
+
+
+def test_cli_missing_arguments_exit_like_the_miniapp():
+    out = subprocess.run([_tool("miniiso_flying_edges"), "-v", "0.5"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0
+    assert "Error: isoval, input_file and output_file must be set." in out.stdout
+
+
+def test_checker_confirms_fe_and_mc_cli_outputs_equivalent(oracle_mod):
+    from miniisosurface_amd import Image3D, save_image
+    p = [g for g in golden_files() if "c1_gaussians_128" in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        for tool, o in (("miniiso_flying_edges", "fe.vtk"), ("miniiso_marching_cubes", "mc.vtk")):
+            subprocess.run([_tool(tool), "-i", "v.vtk", "-o", o, "-v", repr(float(iso)), "-y", "y/r.yaml"], cwd=tmp,
+                           check=True, capture_output=True, timeout=300)
+        out = subprocess.run([_tool("miniiso_same_contents"), "fe.vtk", "mc.vtk"], cwd=tmp, capture_output=True,
+                             text=True, timeout=120)
+        assert "The two meshes are equivalent." in out.stdout

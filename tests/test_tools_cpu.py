@@ -1,0 +1,164 @@
+"""CPU: mesh file format, the order-independent mesh checker (Python and C++), and the C ABI
+exports. The reference's own checker and drivers (compiled into oracle/_ref) pin both where
+present."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from helpers import golden_files, load_fixture
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _mesh(oracle_mod, path="rand20_iso050"):
+    p = [g for g in golden_files() if path in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    from miniisosurface_amd import TriangleMesh
+    return vol, iso, TriangleMesh(d["fe_points"], d["fe_normals"], d["fe_triangles"]), \
+        TriangleMesh(d["mc_points"], d["mc_normals"], d["mc_triangles"])
+
+
+def _shuffled(m, seed=0):
+    from miniisosurface_amd import TriangleMesh
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(m.points.shape[0])
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    tris = inv[m.triangles.astype(np.int64)]
+    tris = tris[rng.permutation(tris.shape[0])]
+    return TriangleMesh(m.points[perm].copy(), m.normals[perm].copy(), tris.astype(np.uint64))
+
+
+def _variants(fe):
+    from miniisosurface_amd import TriangleMesh
+    moved = fe.points.copy()
+    moved[3, 1] = np.nextafter(moved[3, 1], np.float32(np.inf))
+    flipped = fe.normals.copy()
+    flipped[5, 0] = -flipped[5, 0] if flipped[5, 0] != 0 else np.float32(1e-3)
+    rolled = fe.triangles.copy()
+    rolled[7] = rolled[7][[1, 2, 0]]
+    return {
+        "shuffled": (_shuffled(fe), True, ""),
+        "moved_point": (TriangleMesh(moved, fe.normals, fe.triangles), False, "point not found"),
+        "changed_normal": (TriangleMesh(fe.points, flipped, fe.triangles), False, "normal not found"),
+        "rotated_triangle": (TriangleMesh(fe.points, fe.normals, rolled), False, "triangle not found"),
+        "dropped_triangle": (TriangleMesh(fe.points, fe.normals, fe.triangles[1:]), False,
+                             "not the same number of triangles"),
+    }
+
+
+def test_checker_rules(oracle_mod):
+    from miniisosurface_amd.meshcheck import same_contents
+    vol, iso, fe, mc = _mesh(oracle_mod)
+    assert same_contents(fe, fe) == (True, "")
+    # FE and MC emit the same triangle stream with different vertex numbering
+    assert same_contents(fe, mc) == (True, "")
+    for name, (m, ok, why) in _variants(fe).items():
+        assert same_contents(fe, m) == (ok, why), name
+    # -0 and +0 compare equal
+    from miniisosurface_amd import TriangleMesh
+    z = TriangleMesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.zeros((3, 3), np.float32),
+                     np.array([[0, 1, 2]], np.uint64))
+    zn = TriangleMesh(-z.points, -z.normals, z.triangles)
+    zn.points[1, 0] = 1.0
+    zn.points[2, 1] = 1.0
+    assert same_contents(z, zn) == (True, "")
+
+
+def _tool(name):
+    from miniisosurface_amd import apps
+    p = apps.tool_path(name)
+    if not os.path.exists(p):
+        apps.build()
+    return p
+
+
+def _run_checker(exe, a, b):
+    out = subprocess.run([exe, a, b], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return "The two meshes are equivalent." in out.stdout
+
+
+def test_cpp_checker_agrees_with_reference_checker(oracle_mod):
+    """Our C++ checker, the Python one and (when built) the reference's SameContentsCheck give
+    the same verdict on every variant."""
+    from miniisosurface_amd import save_mesh
+    from miniisosurface_amd.meshcheck import same_contents
+    vol, iso, fe, mc = _mesh(oracle_mod)
+    ours = _tool("miniiso_same_contents")
+    ref = oracle_mod.ref_binary("SameContentsCheck")
+    with tempfile.TemporaryDirectory() as tmp:
+        pa = os.path.join(tmp, "a.vtk")
+        save_mesh(pa, fe)
+        cases = dict(_variants(fe))
+        cases["mc"] = (mc, True, "")
+        for name, (m, ok, _) in cases.items():
+            pb = os.path.join(tmp, name + ".vtk")
+            save_mesh(pb, m)
+            assert _run_checker(ours, pa, pb) == ok, name
+            assert same_contents(fe, m)[0] == ok, name
+            if ref:
+                assert _run_checker(ref, pa, pb) == ok, name
+
+
+def test_mesh_writer_matches_reference_output(oracle_mod):
+    """save_mesh writes the bytes the reference's saveTriangleMesh writes for the same mesh."""
+    ref = oracle_mod.ref_binary("flyingEdgesSerial")
+    if not ref:
+        pytest.skip("reference binaries not built here (oracle/_ref)")
+    from miniisosurface_amd import Image3D, load_mesh, save_image, save_mesh
+    vol, iso, fe, mc = _mesh(oracle_mod, "spacing_origin_24x19x14")
+    p = [g for g in golden_files() if "spacing_origin_24x19x14" in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        subprocess.run([ref, "-i", "v.vtk", "-o", "ref.vtk", "-v", repr(float(iso)), "-y", "y/r.yaml"], cwd=tmp,
+                       check=True, capture_output=True, timeout=120)
+        save_mesh(os.path.join(tmp, "ours.vtk"), fe)
+        a = open(os.path.join(tmp, "ref.vtk"), "rb").read()
+        b = open(os.path.join(tmp, "ours.vtk"), "rb").read()
+        assert a == b
+        back = load_mesh(os.path.join(tmp, "ref.vtk"))
+        assert np.array_equal(back.triangles, fe.triangles)
+
+
+@pytest.mark.parametrize("vtk_type", ["char", "short", "unsigned_short", "int", "float", "double"])
+def test_image_round_trip_types(vtk_type):
+    from miniisosurface_amd import Image3D, load_image, save_image
+    rng = np.random.default_rng(1)
+    vol = rng.integers(-100, 100, (5, 6, 7)).astype(np.float32)
+    if vtk_type == "unsigned_short":
+        vol = np.abs(vol)
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "v.vtk")
+        save_image(p, Image3D(vol, (0.5, 2.0, 1.25), (-1.0, 0.0, 3.5)), vtk_type=vtk_type)
+        img = load_image(p)
+    assert img.dims == (7, 6, 5)
+    assert np.array_equal(img.data, vol)
+    assert img.spacing == (0.5, 2.0, 1.25) and img.origin == (-1.0, 0.0, 3.5)
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "miniiso.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(miso_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    """The built library exports every function include/miniiso.h declares (no GPU call)."""
+    from miniisosurface_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from miniisosurface_amd import build
+        build.build()
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _header_symbols()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    L.miso_version.restype = ctypes.c_char_p
+    assert L.miso_version().startswith(b"miniiso")
