@@ -112,3 +112,24 @@ def test_checker_confirms_fe_and_mc_cli_outputs_equivalent(oracle_mod):
         out = subprocess.run([_tool("miniiso_same_contents"), "fe.vtk", "mc.vtk"], cwd=tmp, capture_output=True,
                              text=True, timeout=120)
         assert "The two meshes are equivalent." in out.stdout
+
+
+@pytest.mark.parametrize("name", ["rand20_iso050", "spacing_origin_24x19x14", "c1_gaussians_128"])
+def test_reference_driver_bound_to_library(oracle_mod, name):
+    """The reference's own serial driver, compiled with include/miniiso.hpp in place of its
+    FlyingEdgesAlgorithm (oracle/Makefile target `integration`), writes the same mesh file as
+    the reference built as is."""
+    bound = oracle_mod.ref_binary("flyingEdgesMiniiso")
+    ref = oracle_mod.ref_binary("flyingEdgesSerial")
+    if not (bound and ref):
+        pytest.skip("reference drivers not built (no reference sources where the tree was built)")
+    from miniisosurface_amd import Image3D, save_image
+    p = [g for g in golden_files() if name in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        for exe, o in ((bound, "bound.vtk"), (ref, "ref.vtk")):
+            r = subprocess.run([exe, "-i", "v.vtk", "-o", o, "-v", repr(float(iso)), "-y", f"y_{o}/r.yaml"], cwd=tmp,
+                               capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stderr
+        assert open(os.path.join(tmp, "bound.vtk"), "rb").read() == open(os.path.join(tmp, "ref.vtk"), "rb").read()
