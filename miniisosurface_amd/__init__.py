@@ -85,12 +85,13 @@ class FlyingEdgesAlgorithm:
         check(lib().miso_fe_create(ctypes.byref(self._vol), ctypes.c_float(self.isoval),
                                    _stream_ptr(stream), ctypes.byref(h)), "FlyingEdgesAlgorithm")
         self._h = h
+        self._destroy = lib().miso_fe_destroy      # kept: module globals may be gone at exit
         self.dims = (self._vol.nx, self._vol.ny, self._vol.nz)
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().miso_fe_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def _pass(self, n):
@@ -180,11 +181,12 @@ class _MC:
         check(lib().miso_mc_create(ctypes.byref(self._vol), ctypes.c_float(float(np.float32(isoval))),
                                    _stream_ptr(stream), ctypes.byref(h)), "MarchingCubes")
         self._h = h
+        self._destroy = lib().miso_mc_destroy
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().miso_mc_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def run(self):

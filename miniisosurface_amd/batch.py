@@ -40,6 +40,7 @@ class BatchExtraction:
         h = ctypes.c_void_p()
         check(lib().miso_batch_create(arr, isos, self.njobs, _stream_ptr(stream), ctypes.byref(h)), "BatchExtraction")
         self._h = h
+        self._destroy = lib().miso_batch_destroy
 
     def run(self):
         check(lib().miso_batch_run(self._h), "batch run")
@@ -88,7 +89,7 @@ class BatchExtraction:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().miso_batch_destroy(self._h)
+            self._destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -22,6 +22,7 @@
 #include "mc_kernels.h"
 #include "fe_rows.h"
 #include "fe_batch.h"
+#include "mc_rows.h"
 
 using namespace miso;
 
@@ -257,8 +258,12 @@ struct McState {
 
     int launch_count(RowState& s)
     {
-        hipLaunchKernelGGL(k_mc_count, dim3(s.row_blocks()), dim3(256), 0, s.stream, (const uint64_t*)s.d_mask,
-                           s.g, (const RowCounts*)s.d_counts, d_owned, d_scan, (int64_t)(1 + ntiles));
+        if (s.lane_rows())
+            hipLaunchKernelGGL(k_mc_count_rows, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                               (const uint64_t*)s.d_mask, s.g, d_owned, d_scan, (int64_t)(1 + ntiles));
+        else
+            hipLaunchKernelGGL(k_mc_count, dim3(s.row_blocks()), dim3(256), 0, s.stream, (const uint64_t*)s.d_mask,
+                               s.g, (const RowCounts*)s.d_counts, d_owned, d_scan, (int64_t)(1 + ntiles));
         HIPCALL(hipGetLastError());
         hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)ntiles), dim3(256), 0, s.stream, (const uint32_t*)d_owned,
                            s.g.ny * s.g.nz, d_moff, d_scan, ntiles, d_total);
@@ -277,6 +282,17 @@ struct McState {
             if (int rc = dev_alloc(&p, (size_t)need)) return rc;
             d_map = p;
             map_bytes = need;
+        }
+        if (s.lane_rows()) {
+            hipLaunchKernelGGL(k_mc_vertices_rows<Index>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                               (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
+                               (const uint64_t*)s.d_voff, (const uint64_t*)d_moff, pts, nrm, (Index*)d_map);
+            HIPCALL(hipGetLastError());
+            hipLaunchKernelGGL(k_mc_triangles_rows<Index>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                               (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,
+                               (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (const Index*)d_map, tris);
+            HIPCALL(hipGetLastError());
+            return MISO_OK;
         }
         hipLaunchKernelGGL(k_mc_vertices<Index>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
                            (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,

@@ -449,15 +449,16 @@ __device__ __forceinline__ uint32_t case_of_half(const uint32_t h[8], int b)
  * gathers of both are in flight together), then the triangles of its cell rows (active cells
  * listed per lane, resolved one cell and then one triangle per lane). */
 /* Lane-row block blk. s_case holds kCaseEdges; vertex ids are written less vbias (the first
- * vertex of this volume's output when several volumes share one buffer). */
-template <typename Index, bool Unit>
+ * vertex of this volume's output when several volumes share one buffer). With Remap, only the
+ * triangles are written, each Flying Edges vertex id replaced by remap[id]. */
+template <typename Index, bool Unit, bool Remap = false>
 __device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ s_case, EmitRowsLds* s_w,
                                                    const float* __restrict__ vol, const uint64_t* __restrict__ mask,
                                                    const Geom& g, float iso, const RowCounts* __restrict__ counts,
                                                    const uint64_t* __restrict__ voff,
                                                    const uint64_t* __restrict__ toff, float* __restrict__ pts,
                                                    float* __restrict__ nrm, Index* __restrict__ tris,
-                                                   uint64_t vbias, int64_t blk)
+                                                   uint64_t vbias, int64_t blk, const Index* __restrict__ remap = nullptr)
 {
     const int lane = lane_id();
     const int wv = threadIdx.x >> 6;
@@ -482,7 +483,7 @@ __device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ 
     const float* __restrict__ plane = vol + k0 * (int64_t)sz;
 
     // ---------------- vertices: rows r0 .. r0 + rpw - 1, [x][y][z] cuts per row
-    {
+    if (!Remap) {
         const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb, has_y ? ((L.m[0] ^ L.m[1]) & vb) : 0ull,
                                  has_z ? ((L.m[0] ^ L.m[2]) & vb) : 0ull};
         const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
@@ -639,7 +640,10 @@ __device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ 
                     const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
                     id[s] = S.base[q][owner] + (uint32_t)__popcll(below);
                 }
-                store_tri<Index>(tris, slot + e, vrel + id[0], vrel + id[1], vrel + id[2]);
+                if (Remap)
+                    store_tri<Index>(tris, slot + e, remap[vrel + id[0]], remap[vrel + id[1]], remap[vrel + id[2]]);
+                else
+                    store_tri<Index>(tris, slot + e, vrel + id[0], vrel + id[1], vrel + id[2]);
             }
             slot += ttot;
             wave_sync();
@@ -662,6 +666,23 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
     __syncthreads();
     fe_emit_rows_block<Index, Unit>(s_case, s_w, vol, mask, g, iso, counts, voff, toff, pts, nrm, tris, 0,
                                     blockIdx.x);
+}
+
+/* Marching Cubes triangles: the Flying Edges triangle stream through the id map. */
+template <typename Index>
+__global__ __launch_bounds__(256) void k_mc_triangles_rows(const uint64_t* __restrict__ mask, Geom g,
+                                                           const RowCounts* __restrict__ counts,
+                                                           const uint64_t* __restrict__ voff,
+                                                           const uint64_t* __restrict__ toff,
+                                                           const Index* __restrict__ fe_to_mc,
+                                                           Index* __restrict__ tris)
+{
+    __shared__ uint64_t s_case[256];
+    __shared__ EmitRowsLds s_w[4];
+    s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
+    __syncthreads();
+    fe_emit_rows_block<Index, true, true>(s_case, s_w, nullptr, mask, g, 0.0f, counts, voff, toff, nullptr, nullptr,
+                                          tris, 0, blockIdx.x, fe_to_mc);
 }
 
 }  // namespace miso
