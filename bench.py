@@ -113,6 +113,37 @@ def read_traffic(kernel):
     return None
 
 
+def bench_c2(reps=10):
+    """512^3 gyroid (8 periods) + U(-0.05, 0.05) noise (seed 0), iso 0: Flying Edges and Marching
+    Cubes, volume resident in HBM, one complete extraction per step."""
+    import torch
+
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+
+    g = synth.gyroid(512, seed=0, device="cuda")
+    dv = m.DeviceVolume.from_tensor(g)
+    n = 512
+    cells = (n - 1) ** 3
+    out = {}
+    for name, algo in (("flying_edges", m.FlyingEdgesAlgorithm(dv, 0.0)), ("marching_cubes", m._MC(dv, 0.0))):
+        for _ in range(3):
+            algo.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            algo.run()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        V, T = algo.counts()
+        b_alg = 4 * n ** 3 + 24 * V + 12 * T
+        out[name] = {"ms_per_step": round(wall * 1e3, 4), "gcells_s": round(cells / wall / 1e9, 2),
+                     "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4), "nverts": V, "ntris": T}
+        if name == "flying_edges":
+            out[name]["pass_ms"] = [round(float(x), 4) for x in algo.pass_times_ms()]
+    return out
+
+
 def bench_batched(reps):
     """256 mixed-size Gaussian volumes (dims log-uniform in [32, 256], 1-6 blobs, iso U[0.2, 0.8],
     seed 0), and one 512^3 gyroid at 16 isovalues in [-1, 1]: the batch (one persistent launch per
@@ -244,6 +275,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--calibrate", action="store_true",
@@ -361,6 +393,8 @@ def main():
         "sparse_iso_0.95": sparse,
         "volume_gen_s": round(gen_s, 2),
     }
+    if not args.no_c2:
+        out["c2_gyroid_512_iso0"] = bench_c2()
     if not args.no_batch:
         out["batched"] = bench_batched(reps=3)
     if not args.no_serve:

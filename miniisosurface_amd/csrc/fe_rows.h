@@ -448,114 +448,144 @@ __device__ __forceinline__ uint32_t case_of_half(const uint32_t h[8], int b)
  * rows (one vertex per lane from an LDS list in output order, two per lane per step so that the
  * gathers of both are in flight together), then the triangles of its cell rows (active cells
  * listed per lane, resolved one cell and then one triangle per lane). */
-/* Lane-row block blk. s_case holds kCaseEdges; vertex ids are written less vbias (the first
- * vertex of this volume's output when several volumes share one buffer). With Remap, only the
- * triangles are written, each Flying Edges vertex id replaced by remap[id]. */
-template <typename Index, bool Unit, bool Remap = false>
-__device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ s_case, EmitRowsLds* s_w,
-                                                   const float* __restrict__ vol, const uint64_t* __restrict__ mask,
-                                                   const Geom& g, float iso, const RowCounts* __restrict__ counts,
-                                                   const uint64_t* __restrict__ voff,
-                                                   const uint64_t* __restrict__ toff, float* __restrict__ pts,
-                                                   float* __restrict__ nrm, Index* __restrict__ tris,
-                                                   uint64_t vbias, int64_t blk, const Index* __restrict__ remap = nullptr)
-{
-    const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
-    EmitRowsLds& S = s_w[wv];
-    const RowLanes rl = row_lanes(g.W);
-    const int64_t R = g.ny * g.nz;
-    const int64_t r0 = (blk * 4 + wv) * rl.rpw;
-    if (r0 >= R) return;
-    const int64_t r = r0 + rl.q;
-    const bool ok = r < R && rl.w < g.W;
-    const int ny = (int)g.ny;
-    const int j = (int)(r % ny), k = (int)(r / ny);
-    const bool has_y = j < ny - 1, has_z = k < (int)g.nz - 1;
-    const LaneWords L = load_lane_words(mask, g, r, rl.w, ok, has_y, has_z);
-    const uint64_t v0 = voff[r0];
-    const uint64_t t0 = toff[r0];
-    const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
-    // the wavefront's first row (j0, k0); voxel offsets below are 32-bit from plane k0
-    const int64_t k0 = r0 / g.ny;
-    const int j0 = (int)(r0 - k0 * g.ny);
-    const uint32_t sz = (uint32_t)(g.nx * g.ny);
-    const float* __restrict__ plane = vol + k0 * (int64_t)sz;
+/* Per-wavefront state of pass 4 in the lane-row layout: rows r0 .. r0 + rpw - 1, lane (q, w)
+ * holding word w of row r0 + q and the words of its three neighbour rows. */
+struct EmitRows {
+    RowLanes rl;
+    int lane, wv;
+    int64_t r0, r;
+    bool ok, has_y, has_z;
+    LaneWords L;
+    uint64_t v0, t0, vb, eb;
+    int64_t k0;   // the wavefront's first row is (j0, k0)
+    int j0;
+};
 
-    // ---------------- vertices: rows r0 .. r0 + rpw - 1, [x][y][z] cuts per row
-    if (!Remap) {
-        const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb, has_y ? ((L.m[0] ^ L.m[1]) & vb) : 0ull,
-                                 has_z ? ((L.m[0] ^ L.m[2]) & vb) : 0ull};
-        const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
-        const uint32_t X = seg_sum(c0, rl.Wp), Y = seg_sum(c1, rl.Wp);
-        const uint32_t lane_tot = c0 + c1 + c2;
-        const uint32_t incl = wave_incl_scan_u32(lane_tot);
-        const uint32_t before_row = __shfl(incl - lane_tot, rl.q * rl.Wp);
-        const uint32_t pos0 = before_row + seg_excl_scan(c0, rl.Wp, rl.w);
-        const uint32_t pos1 = before_row + X + seg_excl_scan(c1, rl.Wp, rl.w);
-        const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
-        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        // two lists, x-edges and then y/z-edges, so that the lanes of one step take the same
-        // load pattern; each entry carries its output position
-        const uint32_t xincl = wave_incl_scan_u32(c0);
-        const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl, 63);
-        const uint32_t xfirst = xincl - c0;
-        const uint32_t yzfirst = (incl - lane_tot) - xfirst;
-        const int wp_log2 = __builtin_ctz((unsigned)rl.Wp);
-        uint32_t* list = S.u.vlist;
+__device__ __forceinline__ bool emit_rows_setup(EmitRows& E, const uint64_t* __restrict__ mask, const Geom& g,
+                                                const uint64_t* __restrict__ voff, const uint64_t* __restrict__ toff,
+                                                int64_t blk)
+{
+    E.lane = lane_id();
+    E.wv = threadIdx.x >> 6;
+    E.rl = row_lanes(g.W);
+    const int64_t R = g.ny * g.nz;
+    E.r0 = (blk * 4 + E.wv) * E.rl.rpw;
+    if (E.r0 >= R) return false;
+    E.r = E.r0 + E.rl.q;
+    E.ok = E.r < R && E.rl.w < g.W;
+    const int ny = (int)g.ny;
+    const int j = (int)(E.r % ny), k = (int)(E.r / ny);
+    E.has_y = j < ny - 1;
+    E.has_z = k < (int)g.nz - 1;
+    E.L = load_lane_words(mask, g, E.r, E.rl.w, E.ok, E.has_y, E.has_z);
+    E.v0 = voff[E.r0];
+    E.t0 = toff[E.r0];
+    E.vb = low_bits(g.nx - 64 * E.rl.w);
+    E.eb = low_bits(g.nx - 1 - 64 * E.rl.w);
+    E.k0 = E.r0 / g.ny;
+    E.j0 = (int)(E.r0 - E.k0 * g.ny);
+    return true;
+}
+
+/* Vertices of the wavefront's rows, [x][y][z] cuts per row, through an LDS list of kEmitVertCap
+ * entries. Voxel offsets are 32-bit from plane k0. */
+template <bool Unit>
+__device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* list, const float* __restrict__ vol,
+                                                   const Geom& g, float iso, float* __restrict__ pts,
+                                                   float* __restrict__ nrm)
+{
+    const RowLanes& rl = E.rl;
+    const LaneWords& L = E.L;
+    const int lane = E.lane;
+    const uint64_t v0 = E.v0, vb = E.vb, eb = E.eb;
+    const bool has_y = E.has_y, has_z = E.has_z;
+    const int j0 = E.j0;
+    const int64_t k0 = E.k0;
+    const float* __restrict__ plane = vol + k0 * (g.nx * g.ny);
+    const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb, has_y ? ((L.m[0] ^ L.m[1]) & vb) : 0ull,
+                             has_z ? ((L.m[0] ^ L.m[2]) & vb) : 0ull};
+    const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
+    const uint32_t X = seg_sum(c0, rl.Wp), Y = seg_sum(c1, rl.Wp);
+    const uint32_t lane_tot = c0 + c1 + c2;
+    const uint32_t incl = wave_incl_scan_u32(lane_tot);
+    const uint32_t before_row = __shfl(incl - lane_tot, rl.q * rl.Wp);
+    const uint32_t pos0 = before_row + seg_excl_scan(c0, rl.Wp, rl.w);
+    const uint32_t pos1 = before_row + X + seg_excl_scan(c1, rl.Wp, rl.w);
+    const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    // two lists, x-edges and then y/z-edges, so that the lanes of one step take the same
+    // load pattern; each entry carries its output position
+    const uint32_t xincl = wave_incl_scan_u32(c0);
+    const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl, 63);
+    const uint32_t xfirst = xincl - c0;
+    const uint32_t yzfirst = (incl - lane_tot) - xfirst;
+    const int wp_log2 = __builtin_ctz((unsigned)rl.Wp);
 #pragma unroll
-        for (int grp = 0; grp < 2; ++grp) {
-            const uint32_t gtotal = grp ? total - xtotal : xtotal;
-            const uint32_t gfirst = grp ? yzfirst : xfirst;
-            for (uint32_t win = 0; win < gtotal; win += kEmitVertCap) {
-                const uint32_t wend = win + kEmitVertCap;
-                uint32_t idx = gfirst;
+    for (int grp = 0; grp < 2; ++grp) {
+        const uint32_t gtotal = grp ? total - xtotal : xtotal;
+        const uint32_t gfirst = grp ? yzfirst : xfirst;
+        for (uint32_t win = 0; win < gtotal; win += kEmitVertCap) {
+            const uint32_t wend = win + kEmitVertCap;
+            uint32_t idx = gfirst;
 #pragma unroll
-                for (int a = grp; a < (grp ? 3 : 1); ++a) {
-                    uint64_t bits = cut[a];
-                    uint32_t p = a == 0 ? pos0 : (a == 1 ? pos1 : pos2);
-                    if (idx + (uint32_t)__popcll(bits) <= win || idx >= wend) {
-                        idx += (uint32_t)__popcll(bits);
-                        continue;
-                    }
-                    while (bits && idx < wend) {
-                        const int b = __builtin_ctzll(bits);
-                        bits &= bits - 1;
-                        if (idx >= win) list[idx - win] = p << 14 | (uint32_t)a << 12 | (uint32_t)lane << 6 | (uint32_t)b;
-                        ++idx;
-                        ++p;
-                    }
+            for (int a = grp; a < (grp ? 3 : 1); ++a) {
+                uint64_t bits = cut[a];
+                uint32_t p = a == 0 ? pos0 : (a == 1 ? pos1 : pos2);
+                if (idx + (uint32_t)__popcll(bits) <= win || idx >= wend) {
                     idx += (uint32_t)__popcll(bits);
+                    continue;
                 }
-                wave_sync();
-                const uint32_t n = min((uint32_t)kEmitVertCap, gtotal - win);
-                for (uint32_t e = lane; e < n; e += 128) {
-                    const bool two = e + 64 < n;
-                    const uint32_t ent0 = list[e];
-                    const uint32_t ent1 = two ? list[e + 64] : ent0;
-                    float p0[3], n0[3], p1[3], n1[3];
-                    emit_vertex<Unit>(plane, g, iso, ent0, j0, (int)k0, wp_log2, grp == 0, p0, n0);
-                    emit_vertex<Unit>(plane, g, iso, ent1, j0, (int)k0, wp_log2, grp == 0, p1, n1);
-                    const uint64_t vid0 = v0 + (ent0 >> 14), vid1 = v0 + (ent1 >> 14);
+                while (bits && idx < wend) {
+                    const int b = __builtin_ctzll(bits);
+                    bits &= bits - 1;
+                    if (idx >= win) list[idx - win] = p << 14 | (uint32_t)a << 12 | (uint32_t)lane << 6 | (uint32_t)b;
+                    ++idx;
+                    ++p;
+                }
+                idx += (uint32_t)__popcll(bits);
+            }
+            wave_sync();
+            const uint32_t n = min((uint32_t)kEmitVertCap, gtotal - win);
+            for (uint32_t e = lane; e < n; e += 128) {
+                const bool two = e + 64 < n;
+                const uint32_t ent0 = list[e];
+                const uint32_t ent1 = two ? list[e + 64] : ent0;
+                float p0[3], n0[3], p1[3], n1[3];
+                emit_vertex<Unit>(plane, g, iso, ent0, j0, (int)k0, wp_log2, grp == 0, p0, n0);
+                emit_vertex<Unit>(plane, g, iso, ent1, j0, (int)k0, wp_log2, grp == 0, p1, n1);
+                const uint64_t vid0 = v0 + (ent0 >> 14), vid1 = v0 + (ent1 >> 14);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    pts[3 * vid0 + c] = p0[c];
+                    nrm[3 * vid0 + c] = n0[c];
+                }
+                if (two) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        pts[3 * vid0 + c] = p0[c];
-                        nrm[3 * vid0 + c] = n0[c];
-                    }
-                    if (two) {
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            pts[3 * vid1 + c] = p1[c];
-                            nrm[3 * vid1 + c] = n1[c];
-                        }
+                        pts[3 * vid1 + c] = p1[c];
+                        nrm[3 * vid1 + c] = n1[c];
                     }
                 }
-                wave_sync();
             }
+            wave_sync();
         }
     }
+}
 
-    // ---------------- triangles: cell rows r0 .. r0 + rpw - 1
+/* Triangles of the wavefront's cell rows: active cells listed per lane in output order, then
+ * resolved one triangle per lane. Ids are written less vbias, or through remap. */
+template <typename Index, bool Remap>
+__device__ __forceinline__ void emit_rows_triangles(const EmitRows& E, EmitRowsLds& S, const uint64_t* __restrict__ s_case,
+                                                    const Geom& g, const RowCounts* __restrict__ counts,
+                                                    const uint64_t* __restrict__ voff, Index* __restrict__ tris,
+                                                    uint64_t vbias, const Index* __restrict__ remap)
+{
+    const RowLanes& rl = E.rl;
+    const LaneWords& L = E.L;
+    const int lane = E.lane;
+    const int64_t r = E.r;
+    const bool ok = E.ok, has_y = E.has_y, has_z = E.has_z;
+    const uint64_t v0 = E.v0, t0 = E.t0, vb = E.vb, eb = E.eb;
     const bool cells = ok && has_y && has_z;
     const CornerWords C = corner_words(L);
     const uint64_t act = cells ? active_cells(C, eb) : 0ull;
@@ -649,6 +679,24 @@ __device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ 
             wave_sync();
         }
     }
+}
+
+/* Lane-row block blk. s_case holds kCaseEdges; vertex ids are written less vbias (the first
+ * vertex of this volume's output when several volumes share one buffer). With Remap, only the
+ * triangles are written, each Flying Edges vertex id replaced by remap[id]. */
+template <typename Index, bool Unit, bool Remap = false>
+__device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ s_case, EmitRowsLds* s_w,
+                                                   const float* __restrict__ vol, const uint64_t* __restrict__ mask,
+                                                   const Geom& g, float iso, const RowCounts* __restrict__ counts,
+                                                   const uint64_t* __restrict__ voff,
+                                                   const uint64_t* __restrict__ toff, float* __restrict__ pts,
+                                                   float* __restrict__ nrm, Index* __restrict__ tris,
+                                                   uint64_t vbias, int64_t blk, const Index* __restrict__ remap = nullptr)
+{
+    EmitRows E;
+    if (!emit_rows_setup(E, mask, g, voff, toff, blk)) return;
+    if (!Remap) emit_rows_vertices<Unit>(E, s_w[E.wv].u.vlist, vol, g, iso, pts, nrm);
+    emit_rows_triangles<Index, Remap>(E, s_w[E.wv], s_case, g, counts, voff, tris, vbias, remap);
 }
 
 template <typename Index, bool Unit>
