@@ -8,8 +8,8 @@
  * contiguous range of the output, and its triangles one contiguous range too.
  *
  *   k_fe_count_rows  pass 1b + 2 (trims, x/y/z cut counts, triangle counts per row)
- *   k_fe_emit_rows   pass 4: vertices (cut edges compacted into an LDS list in output order, one
- *                    vertex per lane, written at consecutive addresses), then triangles (active
+ *   k_fe_emit_rows   pass 4: vertices (cut edges compacted into LDS lists, x-edges then y/z-edges,
+ *                    one vertex per lane, written at consecutive addresses), then triangles (active
  *                    cells listed in output order, resolved one triangle per lane, vertex ids from
  *                    row offsets plus popcount ranks held in LDS)
  */
@@ -445,9 +445,9 @@ __device__ __forceinline__ uint32_t case_of_half(const uint32_t h[8], int b)
 }
 
 /* Pass 4 for the (row, word) lane layout in one launch: each wavefront writes the vertices of its
- * rows (one vertex per lane from an LDS list in output order, two per lane per step so that the
- * gathers of both are in flight together), then the triangles of its cell rows (active cells
+ * rows (one vertex per lane from LDS lists), then the triangles of its cell rows (active cells
  * listed per lane, resolved one cell and then one triangle per lane). */
+
 /* Per-wavefront state of pass 4 in the lane-row layout: rows r0 .. r0 + rpw - 1, lane (q, w)
  * holding word w of row r0 + q and the words of its three neighbour rows. */
 struct EmitRows {
@@ -546,25 +546,15 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
             }
             wave_sync();
             const uint32_t n = min((uint32_t)kEmitVertCap, gtotal - win);
-            for (uint32_t e = lane; e < n; e += 128) {
-                const bool two = e + 64 < n;
-                const uint32_t ent0 = list[e];
-                const uint32_t ent1 = two ? list[e + 64] : ent0;
-                float p0[3], n0[3], p1[3], n1[3];
-                emit_vertex<Unit>(plane, g, iso, ent0, j0, (int)k0, wp_log2, grp == 0, p0, n0);
-                emit_vertex<Unit>(plane, g, iso, ent1, j0, (int)k0, wp_log2, grp == 0, p1, n1);
-                const uint64_t vid0 = v0 + (ent0 >> 14), vid1 = v0 + (ent1 >> 14);
+            for (uint32_t e = lane; e < n; e += 64) {
+                const uint32_t ent = list[e];
+                float pp[3], nn[3];
+                emit_vertex<Unit>(plane, g, iso, ent, j0, (int)k0, wp_log2, grp == 0, pp, nn);
+                const uint64_t vid = v0 + (ent >> 14);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    pts[3 * vid0 + c] = p0[c];
-                    nrm[3 * vid0 + c] = n0[c];
-                }
-                if (two) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        pts[3 * vid1 + c] = p1[c];
-                        nrm[3 * vid1 + c] = n1[c];
-                    }
+                    pts[3 * vid + c] = pp[c];
+                    nrm[3 * vid + c] = nn[c];
                 }
             }
             wave_sync();
