@@ -54,6 +54,18 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* Unaligned (4-byte aligned) loads of 2, 3 or 4 consecutive voxels. */
+struct Vox2 { float x, y; };
+struct Vox3 { float x, y, z; };
+struct Vox4 { float x, y, z, w; };
+template <typename T>
+__device__ __forceinline__ T load_vox(const float* __restrict__ p)
+{
+    T t;
+    __builtin_memcpy(&t, p, sizeof(T));
+    return t;
+}
+
 /* bits [0, count) of a 64-bit word, count clipped to [0, 64] */
 __device__ __forceinline__ uint64_t low_bits(int64_t count)
 {

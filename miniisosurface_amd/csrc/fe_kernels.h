@@ -29,7 +29,10 @@ constexpr uint64_t kFlagPre = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
 
 /* ------------------------------------------------------------------ pass 1a: sign mask */
-/* rows 4 blk .. 4 blk + 3, one per wavefront */
+/* Rows 4 blk .. 4 blk + 3, one per wavefront. Each lane loads 4 consecutive voxels (one 16-byte
+ * load, so a wavefront step covers 256 voxels = 4 mask words); its 4 sign bits are placed at
+ * bit 4 * (lane % 16) and OR-reduced over each 16-lane row with DPP, so lane 16 t + 15 ends up
+ * holding mask word t of the step. Voxels past nx read as below the isovalue. */
 __device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol, const Geom& g, float iso,
                                                   uint64_t* __restrict__ mask, int64_t blk)
 {
@@ -38,18 +41,41 @@ __device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol,
     if (r >= g.ny * g.nz) return;
     const float* __restrict__ row = vol + r * g.nx;
     uint64_t* __restrict__ mrow = mask + r * g.W;
-    for (int64_t w0 = 0; w0 < g.W; w0 += 4) {
-        float v[4];
+    const int sub = lane & 15;
+    constexpr int kSteps = 4;                      // 1024 voxels in flight per wavefront
+    for (int64_t c0 = 0; c0 < g.nx; c0 += 256 * kSteps) {
+        uint32_t nib[kSteps];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t i = (w0 + u) * 64 + lane;
-            v[u] = i < g.nx ? row[i] : 0.0f;
+        for (int u = 0; u < kSteps; ++u) {
+            const int64_t i = c0 + 256 * u + 4 * lane;
+            float v[4];
+            if (i + 4 <= g.nx) {
+                const Vox4 q = load_vox<Vox4>(row + i);
+                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) v[c] = i + c < g.nx ? row[i + c] : 0.0f;
+            }
+            uint32_t b = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b |= (uint32_t)(i + c < g.nx && v[c] >= iso) << c;
+            nib[u] = b;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int64_t i = (w0 + u) * 64 + lane;
-            const uint64_t b = __ballot(i < g.nx && v[u] >= iso);
-            if (lane == 0 && w0 + u < g.W) mrow[w0 + u] = b;
+        for (int u = 0; u < kSteps; ++u) {
+            // bits of lane sub go to 4 sub .. 4 sub + 3 of the (lo, hi) 32-bit halves of the word
+            uint32_t lo = sub < 8 ? nib[u] << (4 * sub) : 0u;
+            uint32_t hi = sub < 8 ? 0u : nib[u] << (4 * (sub - 8));
+            lo |= dpp_u32<kDppRowShr1>(lo);
+            hi |= dpp_u32<kDppRowShr1>(hi);
+            lo |= dpp_u32<kDppRowShr2>(lo);
+            hi |= dpp_u32<kDppRowShr2>(hi);
+            lo |= dpp_u32<kDppRowShr4>(lo);
+            hi |= dpp_u32<kDppRowShr4>(hi);
+            lo |= dpp_u32<kDppRowShr8>(lo);
+            hi |= dpp_u32<kDppRowShr8>(hi);
+            const int64_t w = (c0 + 256 * u) / 64 + (lane >> 4);
+            if (sub == 15 && w < g.W) mrow[w] = (uint64_t)hi << 32 | lo;
         }
     }
 }

@@ -303,18 +303,6 @@ __device__ __forceinline__ void fe_edge_point(const float* __restrict__ v, const
     nrm[2] = ga2 + w * (gb2 - ga2);
 }
 
-/* Unaligned (4-byte aligned) loads of 2, 3 or 4 consecutive voxels. */
-struct Vox2 { float x, y; };
-struct Vox3 { float x, y, z; };
-struct Vox4 { float x, y, z, w; };
-template <typename T>
-__device__ __forceinline__ T load_vox(const float* __restrict__ p)
-{
-    T t;
-    __builtin_memcpy(&t, p, sizeof(T));
-    return t;
-}
-
 /* interior central difference (lower - upper) / (2 h), exact as * 0.5 for unit spacing */
 template <bool Unit>
 __device__ __forceinline__ float central(float lo, float hi, float s)
