@@ -501,12 +501,17 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
     const uint32_t pos1 = before_row + X + seg_excl_scan(c1, rl.Wp, rl.w);
     const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-    // two lists, x-edges and then y/z-edges, so that the lanes of one step take the same
-    // load pattern; each entry carries its output position
-    const uint32_t xincl = wave_incl_scan_u32(c0);
-    const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl, 63);
-    const uint32_t xfirst = xincl - c0;
-    const uint32_t yzfirst = (incl - lane_tot) - xfirst;
+    // Two lists, x-edges and then y/z-edges, so that the lanes of one step take the same load
+    // pattern; each entry carries its output position. Within a list, entries run word-major
+    // (word w of all the wavefront's rows, then word w + 1), so that one step's gathers fall in
+    // a narrow x-range of neighbouring rows and share cache lines.
+    const int tl = rl.w * rl.rpw + rl.q;                              // word-major lane order
+    const int from = (lane % rl.rpw) * rl.Wp + lane / rl.rpw;         // lane at word-major position `lane`
+    const uint32_t c0t = (uint32_t)__shfl((int)c0, from), c12t = (uint32_t)__shfl((int)(c1 + c2), from);
+    const uint32_t xincl_t = wave_incl_scan_u32(c0t), yzincl_t = wave_incl_scan_u32(c12t);
+    const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl_t, 63);
+    const uint32_t xfirst = (uint32_t)__shfl((int)(xincl_t - c0t), tl);
+    const uint32_t yzfirst = (uint32_t)__shfl((int)(yzincl_t - c12t), tl);
     const int wp_log2 = __builtin_ctz((unsigned)rl.Wp);
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp) {
