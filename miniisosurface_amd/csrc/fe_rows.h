@@ -611,27 +611,32 @@ __device__ __forceinline__ void emit_rows_triangles(const EmitRows& E, EmitRowsL
     uint64_t slot = t0;
     for (uint32_t win = 0; win < ctotal; win += kEmitCellCap) {
         const uint32_t wend = win + kEmitCellCap;
+        // each lane lists the positions (lane, bit) of its active cells
         if (ncell && cfirst < wend && cfirst + ncell > win) {
             uint32_t p = cfirst;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                uint32_t a = h ? (uint32_t)(act >> 32) : (uint32_t)act;
-                while (a && p < wend) {
-                    const int b = __builtin_ctz(a);
-                    a &= a - 1;
-                    if (p >= win) {
-                        const uint32_t cs = case_of_half(h ? chi : clo, b);
-                        S.u.t.cells[p - win] = (uint32_t)lane | (uint32_t)(b + 32 * h) << 6 | cs << 12;
-                    }
-                    ++p;
-                }
+            for (uint64_t a = act; a && p < wend;) {
+                const int b = __builtin_ctzll(a);
+                a &= a - 1;
+                if (p >= win) S.u.t.cells[p - win] = (uint32_t)lane | (uint32_t)b << 6;
+                ++p;
             }
         }
         wave_sync();
         const uint32_t n = min((uint32_t)kEmitCellCap, ctotal - win);
         for (uint32_t g0 = 0; g0 < n; g0 += 64) {
+            // one cell per lane: its case from the owner lane's corner words (read across lanes)
             const uint32_t idx = g0 + (uint32_t)lane;
-            const uint32_t ent = idx < n ? S.u.t.cells[idx] : 0u;
+            uint32_t ent = idx < n ? S.u.t.cells[idx] : 0u;
+            {
+                const int src = (int)(ent & 63), b = (int)((ent >> 6) & 63);
+                uint32_t half[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t l = (uint32_t)__shfl((int)clo[q], src), h = (uint32_t)__shfl((int)chi[q], src);
+                    half[q] = b >= 32 ? h : l;
+                }
+                ent |= case_of_half(half, b & 31) << 12;
+            }
             const uint32_t nt = idx < n ? (uint32_t)case_tri_count(s_case[ent >> 12]) : 0u;
             const uint32_t tincl = wave_incl_scan_u32(nt);
             const uint32_t ttot = (uint32_t)__builtin_amdgcn_readlane((int)tincl, 63);
