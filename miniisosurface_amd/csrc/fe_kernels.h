@@ -32,7 +32,8 @@ constexpr uint64_t kValMask = (1ull << 62) - 1;
 /* Rows 4 blk .. 4 blk + 3, one per wavefront. Each lane loads 4 consecutive voxels (one 16-byte
  * load, so a wavefront step covers 256 voxels = 4 mask words); its 4 sign bits are placed at
  * bit 4 * (lane % 16) and OR-reduced over each 16-lane row with DPP, so lane 16 t + 15 ends up
- * holding mask word t of the step. Voxels past nx read as below the isovalue. */
+ * holding mask word t of the step. Voxels past nx read as below the isovalue. Rows of at most
+ * 256 voxels take one voxel per lane and __ballot instead (fewer instructions per short row). */
 __device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol, const Geom& g, float iso,
                                                   uint64_t* __restrict__ mask, int64_t blk)
 {
@@ -41,12 +42,29 @@ __device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol,
     if (r >= g.ny * g.nz) return;
     const float* __restrict__ row = vol + r * g.nx;
     uint64_t* __restrict__ mrow = mask + r * g.W;
+    if (g.W <= 4) {
+        // short rows: one 4-byte voxel per lane and word, the sign bits straight from __ballot
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t i = 64 * u + lane;
+            v[u] = (u < g.W && i < g.nx) ? row[i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t b = __ballot(64 * u + lane < g.nx && v[u] >= iso);
+            if (lane == 0 && u < g.W) mrow[u] = b;
+        }
+        return;
+    }
     const int sub = lane & 15;
     constexpr int kSteps = 4;                      // 1024 voxels in flight per wavefront
     for (int64_t c0 = 0; c0 < g.nx; c0 += 256 * kSteps) {
         uint32_t nib[kSteps];
 #pragma unroll
         for (int u = 0; u < kSteps; ++u) {
+            nib[u] = 0;
+            if (c0 + 256 * u >= g.nx) continue;    // wavefront-uniform: short rows skip whole steps
             const int64_t i = c0 + 256 * u + 4 * lane;
             float v[4];
             if (i + 4 <= g.nx) {
@@ -63,6 +81,7 @@ __device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol,
         }
 #pragma unroll
         for (int u = 0; u < kSteps; ++u) {
+            if (c0 + 256 * u >= g.nx) break;
             // bits of lane sub go to 4 sub .. 4 sub + 3 of the (lo, hi) 32-bit halves of the word
             uint32_t lo = sub < 8 ? nib[u] << (4 * sub) : 0u;
             uint32_t hi = sub < 8 ? 0u : nib[u] << (4 * (sub - 8));
