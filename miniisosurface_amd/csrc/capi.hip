@@ -284,9 +284,14 @@ struct McState {
             map_bytes = need;
         }
         if (s.lane_rows()) {
-            hipLaunchKernelGGL(k_mc_vertices_rows<Index>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,
-                               (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
-                               (const uint64_t*)s.d_voff, (const uint64_t*)d_moff, pts, nrm, (Index*)d_map);
+            if (s.unit_geometry())
+                hipLaunchKernelGGL((k_mc_vertices_rows<Index, true>), dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                                   s.d_vol, (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
+                                   (const uint64_t*)s.d_voff, (const uint64_t*)d_moff, pts, nrm, (Index*)d_map);
+            else
+                hipLaunchKernelGGL((k_mc_vertices_rows<Index, false>), dim3(s.lane_row_blocks()), dim3(256), 0,
+                                   s.stream, s.d_vol, (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
+                                   (const uint64_t*)s.d_voff, (const uint64_t*)d_moff, pts, nrm, (Index*)d_map);
             HIPCALL(hipGetLastError());
             hipLaunchKernelGGL(k_mc_triangles_rows<Index>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
                                (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,

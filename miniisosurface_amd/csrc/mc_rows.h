@@ -127,7 +127,7 @@ __device__ __forceinline__ void mc_corners(const Geom& g, int64_t i, int64_t j, 
     B[a] = A[a] + g.sp[a];
 }
 
-template <typename Index>
+template <typename Index, bool Unit>
 __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restrict__ vol,
                                                           const uint64_t* __restrict__ mask, Geom g, float iso,
                                                           const RowCounts* __restrict__ counts,
@@ -256,9 +256,22 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
                     ++kk;
                 }
                 const int64_t i = 64 * (int64_t)w + b;
-                float A[3], B[3], pp[3], nn[3];
-                mc_corners(g, i, jj, kk, e, A, B);
-                edge_point(vol, g, iso, edge_axis(e), i + edge_dx(e), jj + edge_dy(e), kk + edge_dz(e), A, B, pp, nn);
+                float pp[3], nn[3];
+                if (Unit) {
+                    // unit spacing, zero origin: the owner's corner coordinates are float(index),
+                    // as Flying Edges' are, so the Flying Edges vertex code gives the same bits
+                    const int ia = (int)i + edge_dx(e), ja = (int)jj + edge_dy(e), ka = (int)kk + edge_dz(e);
+                    const float* __restrict__ v = vol + ((int64_t)ka * g.ny + ja) * g.nx + ia;
+                    if (edge_axis(e) == 0)
+                        vertex_x<true>(v, g, iso, ia, ja, ka, pp, nn);
+                    else
+                        vertex_yz<true>(v, g, iso, edge_axis(e), ia, ja, ka, pp, nn);
+                } else {
+                    float A[3], B[3];
+                    mc_corners(g, i, jj, kk, e, A, B);
+                    edge_point(vol, g, iso, edge_axis(e), i + edge_dx(e), jj + edge_dy(e), kk + edge_dz(e), A, B, pp,
+                               nn);
+                }
                 const uint64_t id = mid + v;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
