@@ -35,6 +35,10 @@ extern "C" {
 
 /* An image volume: data[i + nx*(j + ny*k)], x fastest, as util::Image3D stores it
  * (flyingEdges/util/Image3D.h:31-37, Image3D.cpp:110-114). */
+/* A device-resident volume is read by kernels on the handle's stream: the caller passes the
+ * stream that produced the data (or one ordered after it). With stream == NULL the library
+ * creates a blocking stream, which is ordered after work on the null stream but not after other
+ * non-blocking streams. */
 typedef struct miso_volume {
     const float* data;       /* host pointer (on_device == 0) or device pointer (on_device == 1) */
     int64_t nx, ny, nz;

@@ -64,7 +64,16 @@ def _volume_struct(image):
     return v, keep
 
 
-def _stream_ptr(stream):
+def _stream_ptr(stream, device_input=False):
+    """The HIP stream the library runs on. With none given, the library's own stream, which is
+    ordered after work on the null stream (torch's default stream); for device-resident input
+    while torch's current stream is another one, that stream, so that the kernels are ordered
+    after the torch work that produced the volume."""
+    if stream is None and device_input:
+        import torch
+        cur = torch.cuda.current_stream()
+        if cur.cuda_stream != 0:
+            stream = cur
     if stream is None:
         return None
     if hasattr(stream, "cuda_stream"):
@@ -83,7 +92,8 @@ class FlyingEdgesAlgorithm:
         self.isoval = float(np.float32(isoval))
         h = ctypes.c_void_p()
         check(lib().miso_fe_create(ctypes.byref(self._vol), ctypes.c_float(self.isoval),
-                                   _stream_ptr(stream), ctypes.byref(h)), "FlyingEdgesAlgorithm")
+                                   _stream_ptr(stream, bool(self._vol.on_device)), ctypes.byref(h)),
+              "FlyingEdgesAlgorithm")
         self._h = h
         self._destroy = lib().miso_fe_destroy      # kept: module globals may be gone at exit
         self.dims = (self._vol.nx, self._vol.ny, self._vol.nz)
@@ -179,7 +189,8 @@ class _MC:
         self._vol, self._keep = _volume_struct(image)
         h = ctypes.c_void_p()
         check(lib().miso_mc_create(ctypes.byref(self._vol), ctypes.c_float(float(np.float32(isoval))),
-                                   _stream_ptr(stream), ctypes.byref(h)), "MarchingCubes")
+                                   _stream_ptr(stream, bool(self._vol.on_device)), ctypes.byref(h)),
+              "MarchingCubes")
         self._h = h
         self._destroy = lib().miso_mc_destroy
 

@@ -38,7 +38,8 @@ class BatchExtraction:
         self.njobs = len(jobs)
         self.dims = [tuple(int(d) for d in vol.dims) for vol, _ in jobs]
         h = ctypes.c_void_p()
-        check(lib().miso_batch_create(arr, isos, self.njobs, _stream_ptr(stream), ctypes.byref(h)), "BatchExtraction")
+        check(lib().miso_batch_create(arr, isos, self.njobs, _stream_ptr(stream, True), ctypes.byref(h)),
+              "BatchExtraction")
         self._h = h
         self._destroy = lib().miso_batch_destroy
 

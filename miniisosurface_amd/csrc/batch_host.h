@@ -204,7 +204,7 @@ int batch_setup(miso_batch* b, void* user_stream)
     if (user_stream) {
         b->stream = (hipStream_t)user_stream;
     } else {
-        HIPCALL(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIPCALL(hipStreamCreateWithFlags(&b->stream, hipStreamDefault));   // ordered after null-stream work
         b->own_stream = true;
     }
     int dev = 0;

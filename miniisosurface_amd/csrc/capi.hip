@@ -118,7 +118,9 @@ struct RowState {
         if (user_stream) {
             stream = (hipStream_t)user_stream;
         } else {
-            HIPCALL(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            // a blocking stream: ordered after work on the null stream (where callers that
+            // produce a device volume without naming a stream usually ran)
+            HIPCALL(hipStreamCreateWithFlags(&stream, hipStreamDefault));
             own_stream = true;
         }
         const int64_t N = g.nx * g.ny * g.nz;

@@ -93,6 +93,9 @@ class IsosurfaceServer:
         for v in self.pool:
             if not isinstance(v, DeviceVolume):
                 raise ValueError("the pool holds DeviceVolume entries")
+        # the slots run on streams of their own: the resident volumes must be complete first
+        import torch
+        torch.cuda.synchronize()
         self.budget = int(budget_bytes)
         self.max_jobs = int(max_jobs)
         self.keep_meshes = keep_meshes

@@ -1,0 +1,78 @@
+"""GPU, full size: the 1024^3 value-noise volume (the benchmark's) checked through properties that
+do not need the oracle to run at that size. Each property is computed independently with torch
+on the GPU from the volume itself:
+
+  - per grid row, the number of cut x-, y- and z-edges (sign changes along each axis) equals the
+    row's vertex counts, and the total is V;
+  - the number of triangles equals the sum over all cells of the case table's triangle count
+    (the case built from the 8 corner signs, numTris from the oracle's table);
+  - the vertex blocks are laid out as Flying Edges lays them out: the x-edge vertices of row
+    (j, k) have y = j and z = k exactly and x in [i, i + 1] for a cut edge at i;
+  - every triangle indexes three distinct vertices below V.
+The whole-mesh bit-exact comparison with the oracle runs at 512^3 (test_fe_c2_gyroid_512)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+
+def test_fe_1024_value_noise_properties(oracle_mod):
+    import torch
+
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+
+    n, iso = 1024, 0.5
+    vol = synth.value_noise(n, seed=0, device="cuda")
+    algo = m.FlyingEdgesAlgorithm(m.DeviceVolume.from_tensor(vol), iso)
+    algo.run()
+    V, T = algo.counts()
+    print(f"[fullsize] V={V} T={T}", flush=True)
+    rows = algo.rows()
+
+    s = vol >= iso                                             # (nz, ny, nx) bool
+    xc = (s[:, :, 1:] != s[:, :, :-1]).sum(dim=2)              # per row (k, j)
+    yc = torch.zeros((n, n), dtype=torch.int64, device="cuda")
+    yc[:, :-1] = (s[:, 1:, :] != s[:, :-1, :]).sum(dim=2)
+    zc = torch.zeros((n, n), dtype=torch.int64, device="cuda")
+    zc[:-1, :] = (s[1:, :, :] != s[:-1, :, :]).sum(dim=2)
+    assert np.array_equal(rows["xcount"].astype(np.int64), xc.reshape(-1).cpu().numpy())
+    assert np.array_equal(rows["ycount"].astype(np.int64), yc.reshape(-1).cpu().numpy())
+    assert np.array_equal(rows["zcount"].astype(np.int64), zc.reshape(-1).cpu().numpy())
+    assert V == int(xc.sum() + yc.sum() + zc.sum())
+
+    # triangles from the case table, plane by plane
+    L = oracle_mod.lib()
+    L.ora_tables_init()
+    ntri = torch.tensor(np.frombuffer((ctypes.c_ubyte * 256).in_dll(L, "ora_num_tris"), dtype=np.uint8).astype(np.int64),
+                        device="cuda")
+    si = s.to(torch.int64)
+    total = 0
+    for k0 in range(0, n - 1, 64):
+        k1 = min(k0 + 64, n - 1)
+        a, b = si[k0:k1], si[k0 + 1:k1 + 1]
+        case = (a[:, :-1, :-1] | a[:, :-1, 1:] << 1 | a[:, 1:, 1:] << 2 | a[:, 1:, :-1] << 3
+                | b[:, :-1, :-1] << 4 | b[:, :-1, 1:] << 5 | b[:, 1:, 1:] << 6 | b[:, 1:, :-1] << 7)
+        total += int(ntri[case].sum())
+    assert T == total
+    del si, s
+
+    mesh = algo.moveOutput()
+    tris = mesh.triangles
+    assert tris.shape == (T, 3)
+    assert int(tris.max()) < V
+    assert np.all(tris[:, 0] != tris[:, 1]) and np.all(tris[:, 1] != tris[:, 2]) and np.all(tris[:, 0] != tris[:, 2])
+    # x-edge vertex blocks: the first xcount vertices of each row
+    R = n * n
+    start = rows["voffset"].astype(np.int64)
+    cnt = rows["xcount"].astype(np.int64)
+    row_of = np.repeat(np.arange(R, dtype=np.int64), cnt)
+    idx = np.repeat(start - np.concatenate([[0], np.cumsum(cnt)[:-1]]), cnt) + np.arange(int(cnt.sum()))
+    p = mesh.points[idx]
+    j, k = row_of % n, row_of // n
+    assert np.array_equal(p[:, 1], j.astype(np.float32))
+    assert np.array_equal(p[:, 2], k.astype(np.float32))
+    assert np.all(np.isfinite(mesh.normals))
+    assert p[:, 0].min() >= 0.0 and p[:, 0].max() <= n - 1
