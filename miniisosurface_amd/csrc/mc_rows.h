@@ -92,11 +92,11 @@ struct McRowsLds {
     uint32_t verts[kMcVertCap];
 };
 
-/* Per case: its cut edges in order of first appearance in the triangle list (nibbles, low
- * first) with their number in the top nibble. */
+/* Per case: for each cut edge e, its rank in order of first appearance in the case's triangle
+ * list (nibble e, bits 4e .. 4e + 3), and the 12-bit mask of cut edges (bits 48 .. 59). */
 __device__ __forceinline__ uint64_t first_appearance(uint64_t packed)
 {
-    uint64_t out = 0;
+    uint64_t pos = 0;
     uint32_t seen = 0;
     int n = 0;
     const int ns = 3 * case_tri_count(packed);
@@ -104,10 +104,10 @@ __device__ __forceinline__ uint64_t first_appearance(uint64_t packed)
         const int e = case_edge(packed, s);
         if ((seen >> e) & 1u) continue;
         seen |= 1u << e;
-        out |= (uint64_t)e << (4 * n);
+        pos |= (uint64_t)n << (4 * e);
         ++n;
     }
-    return out | (uint64_t)n << 60;
+    return pos | (uint64_t)seen << 48;
 }
 
 /* Marching Cubes corner coordinates of the owner cell (marchingCubes/util/Image3D.cpp:22-65):
@@ -227,20 +227,20 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
                     ++kk;
                 }
                 order = s_first[ent >> 12];
-                uint32_t cutm = 0;
-                const int ne = (int)(order >> 60);
-                for (int s = 0; s < ne; ++s) cutm |= 1u << ((order >> (4 * s)) & 0xF);
-                own = cutm & mc_owned_edges(w == 0 && b == 0, jj == 0, kk == 0);
+                own = (uint32_t)(order >> 48) & 0xFFFu & mc_owned_edges(w == 0 && b == 0, jj == 0, kk == 0);
             }
             const uint32_t no = (uint32_t)__popc(own);
             const uint32_t vincl = wave_incl_scan_u32(no);
             const uint32_t vtot = (uint32_t)__builtin_amdgcn_readlane((int)vincl, 63);
             if (own) {
-                uint32_t pos = vincl - no;
-                const int ne = (int)(order >> 60);
-                for (int s = 0; s < ne; ++s) {
-                    const int e = (int)((order >> (4 * s)) & 0xF);
-                    if ((own >> e) & 1u) S.verts[pos++] = (uint32_t)lane | (uint32_t)e << 6;
+                // owned edges in first-appearance order: rank = owned edges appearing earlier
+                uint32_t posmask = 0;
+                for (uint32_t o = own; o; o &= o - 1) posmask |= 1u << ((order >> (4 * __builtin_ctz(o))) & 0xF);
+                const uint32_t base = vincl - no;
+                for (uint32_t o = own; o; o &= o - 1) {
+                    const int e = __builtin_ctz(o);
+                    const uint32_t p = (uint32_t)(order >> (4 * e)) & 0xF;
+                    S.verts[base + (uint32_t)__popc(posmask & ((1u << p) - 1u))] = (uint32_t)lane | (uint32_t)e << 6;
                 }
             }
             wave_sync();
