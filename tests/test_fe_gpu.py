@@ -62,6 +62,11 @@ def _cases():
     yield "rand_130x7x5_sp", rng.uniform(0, 1, (5, 7, 130)).astype(np.float32), 0.42, (0.3, 1.7, 2.2), (-4.0, 1.0, 3.5)
     yield "rand_1030x4x3", rng.uniform(0, 1, (3, 4, 1030)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "rand_2100x3x2", rng.uniform(0, 1, (2, 3, 2100)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
+    # rows of 64 mask words (the widest the (row, word) lane kernels take) and just wider (the
+    # per-row kernels)
+    yield "rand_4096x3x3", rng.uniform(0, 1, (3, 3, 4096)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
+    yield "rand_4097x3x2", rng.uniform(0, 1, (2, 3, 4097)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
+    yield "rand_5000x4x3_sp", rng.uniform(0, 1, (3, 4, 5000)).astype(np.float32), 0.4, (0.5, 2.0, 1.5), (1.0, -2.0, 0.25)
     yield "checker_64", (np.indices((64, 64, 64)).sum(0) % 2).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "gauss_96x80x72", synth.gaussians((96, 80, 72), 6, 3, sigma_range=(5.0, 14.0)), 0.3, (1, 1, 1), (0, 0, 0)
     yield "gyroid_128", synth.gyroid(128, 1), 0.0, (1, 1, 1), (0, 0, 0)
