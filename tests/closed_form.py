@@ -196,3 +196,39 @@ def mc_closed_form(vol, iso, spacing=(1, 1, 1), origin=(0, 0, 0), table=None):
                     tris.append(tri)
     return (np.array(pts, F).reshape(-1, 3), np.array(nrm, F).reshape(-1, 3),
             np.array(tris, np.uint64).reshape(-1, 3))
+
+
+def fe_fp64(vol, iso, sp=(1, 1, 1), org=(0, 0, 0)):
+    """Flying Edges points and normals evaluated in float64 (the exact grid coordinates
+    origin + index * spacing, float64 interpolation weight and gradients) in Flying Edges vertex
+    order: the yardstick for the float32 error that the reference's own arithmetic carries."""
+    vol32 = np.ascontiguousarray(vol, np.float32); vol64 = vol32.astype(np.float64)
+    nz, ny, nx = vol.shape
+    s = vol32 >= np.float32(iso)
+    cut=[np.zeros((nz,ny,nx),bool) for _ in range(3)]
+    cut[0][:,:,:-1]=s[:,:,:-1]!=s[:,:,1:]; cut[1][:,:-1,:]=s[:,:-1,:]!=s[:,1:,:]; cut[2][:-1,:,:]=s[:-1,:,:]!=s[1:,:,:]
+    cnt=np.stack([c.sum(axis=2) for c in cut],axis=-1); per_row=cnt.sum(axis=-1).reshape(-1)
+    row_off=np.concatenate([[0],np.cumsum(per_row)])[:-1].reshape(nz,ny)
+    V=int(per_row.sum()); P=np.zeros((V,3)); N=np.zeros((V,3))
+    def g64(i,j,k):
+        out=[]
+        for a,(n,idx) in enumerate(((nx,i),(ny,j),(nz,k))):
+            st=[0,0,0]; st[a]=1
+            hi_i=np.minimum(idx+1,n-1); lo_i=np.maximum(idx-1,0)
+            ii=[i,j,k]; ih=list(ii); il=list(ii); ih[a]=hi_i; il[a]=lo_i
+            hi=vol64[ih[2],ih[1],ih[0]]; lo=vol64[il[2],il[1],il[0]]
+            run=np.where((idx==0)|(idx==n-1),1.0,2.0)*sp[a]
+            out.append((lo-hi)/run)
+        return out
+    for a in range(3):
+        rank=np.cumsum(cut[a],axis=2)-1; block=row_off+(cnt[:,:,:a].sum(axis=-1) if a else 0)
+        k,j,i=np.nonzero(cut[a]); vid=(block[:,:,None]+rank)[k,j,i]
+        st=[0,0,0]; st[a]=1
+        v0=vol64[k,j,i]; v1=vol64[k+st[2],j+st[1],i+st[0]]
+        w=(iso-v0)/(v1-v0)
+        A=[org[0]+i*sp[0], org[1]+j*sp[1], org[2]+k*sp[2]]
+        gA=g64(i,j,k); gB=g64(i+st[0],j+st[1],k+st[2])
+        for c in range(3):
+            B=A[c]+(sp[c] if c==a else 0.0)
+            P[vid,c]=A[c]+w*(B-A[c]); N[vid,c]=gA[c]+w*(gB[c]-gA[c])
+    return P,N

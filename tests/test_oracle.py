@@ -139,27 +139,25 @@ def test_oracle_mc_matches_owner_rule(oracle_mod, name, vol, iso, sp, org):
     assert bits_equal(r.triangles, tris)
 
 
-def test_oracle_points_against_fp64(oracle_mod):
-    """Reports the FP32 interpolation error against an FP64 evaluation of the same formulas."""
-    vol = np.random.default_rng(5).uniform(0, 1, (16, 16, 16)).astype(np.float32)
-    r = oracle_mod.FEResult(vol, 0.5)
-    # recompute x-edge points in float64 from the crossing's endpoint values
-    s = vol >= np.float32(0.5)
-    k, j, i = np.nonzero(s[:, :, :-1] != s[:, :, 1:])
-    v0 = vol[k, j, i].astype(np.float64)
-    v1 = vol[k, j, i + 1].astype(np.float64)
-    x64 = i + (0.5 - v0) / (v1 - v0)
-    # x-edge vertices of each row come first in that row's block
-    xs = []
-    rows = r.rows
-    for kk in range(16):
-        for jj in range(16):
-            rr = kk * 16 + jj
-            n = int(rows["xcount"][rr])
-            xs.append(r.points[int(rows["xstart"][rr]):int(rows["xstart"][rr]) + n, 0])
-    x32 = np.concatenate(xs).astype(np.float64)
-    err = np.abs(x32 - x64).max()
-    assert err < 1e-5, err
+@pytest.mark.parametrize("name,vol,iso,sp,org", [
+    ("rand_32", np.random.default_rng(5).uniform(0, 1, (32, 32, 32)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)),
+    ("gauss_64", None, 0.5, (1, 1, 1), (0, 0, 0)),
+    ("gyroid_48_sp", None, 0.0, (0.5, 0.75, 1.25), (1.0, -2.0, 3.0))], ids=lambda x: x if isinstance(x, str) else "")
+def test_oracle_points_against_fp64(oracle_mod, name, vol, iso, sp, org):
+    """Reports the float32 error of points and normals (the reference's, and therefore the GPU's,
+    to the bit) against a float64 evaluation of the same formulas. Points stay within a few ULP of
+    their coordinate (the n-1 coordinate of Flying Edges is (origin + (n-2) s) + s in float32);
+    normal components can be near zero, so their bound is absolute."""
+    from miniisosurface_amd import synth
+    if vol is None:
+        vol = synth.gaussians(64, 3, 1, sigma_range=(5.0, 12.0)) if name.startswith("gauss") else synth.gyroid(48, 1)
+    r = oracle_mod.FEResult(vol, iso, sp, org, want_debug=False)
+    P, N = cf.fe_fp64(vol, iso, sp, org)
+    ulp = np.abs(r.points.astype(np.float64) - P) / np.spacing(np.abs(P).astype(np.float32)).astype(np.float64)
+    perr, nerr = float(np.abs(r.points - P).max()), float(np.abs(r.normals - N).max())
+    print(f"[fp64] {name}: V={r.nverts} points max abs {perr:.3g} ({ulp.max():.2f} ulp), normals max abs {nerr:.3g}")
+    assert ulp.max() < 16
+    assert nerr < 1e-6
 
 
 def test_reference_binaries_agree_with_oracle(oracle_mod):
