@@ -208,10 +208,11 @@ __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__
     uint32_t first = trim ? (uint32_t)(64 * rl.w + __builtin_ctzll(trim)) : 0xFFFFFFFFu;
     uint32_t lastp1 = trim ? (uint32_t)(64 * rl.w + 64 - __builtin_clzll(trim)) : 0u;
     uint32_t tc = (has_y && has_z) ? word_tri_count(corner_words(L), eb) : 0u;
-    const uint32_t xc = seg_sum((uint32_t)__popcll(xcut), rl.Wp);
-    const uint32_t yc = seg_sum((uint32_t)__popcll(ycut), rl.Wp);
-    const uint32_t zc = seg_sum((uint32_t)__popcll(zcut), rl.Wp);
-    tc = seg_sum(tc, rl.Wp);
+    // two counts per 32-bit scan: row totals stay below 2^16 (nx <= 4096, <= 5 triangles a cell)
+    const uint32_t xy = seg_sum((uint32_t)__popcll(xcut) | (uint32_t)__popcll(ycut) << 16, rl.Wp);
+    const uint32_t zt = seg_sum((uint32_t)__popcll(zcut) | tc << 16, rl.Wp);
+    const uint32_t xc = xy & 0xFFFFu, yc = xy >> 16, zc = zt & 0xFFFFu;
+    tc = zt >> 16;
     first = seg_reduce<OpMin>(first, rl.Wp);
     lastp1 = seg_reduce<OpMax>(lastp1, rl.Wp);
     if (r < R && rl.w == 0) {
@@ -493,12 +494,17 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
     const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb, has_y ? ((L.m[0] ^ L.m[1]) & vb) : 0ull,
                              has_z ? ((L.m[0] ^ L.m[2]) & vb) : 0ull};
     const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
-    const uint32_t X = seg_sum(c0, rl.Wp), Y = seg_sum(c1, rl.Wp);
+    // x and y counts share one segmented scan (row totals below 2^16)
+    const uint32_t p01 = c0 | c1 << 16;
+    const uint32_t inc01 = seg_incl<OpAdd>(p01, rl.Wp, rl.w);
+    const uint32_t tot01 = (uint32_t)__shfl((int)inc01, (lane & ~(rl.Wp - 1)) + rl.Wp - 1);
+    const uint32_t ex01 = inc01 - p01;
+    const uint32_t X = tot01 & 0xFFFFu, Y = tot01 >> 16;
     const uint32_t lane_tot = c0 + c1 + c2;
     const uint32_t incl = wave_incl_scan_u32(lane_tot);
     const uint32_t before_row = __shfl(incl - lane_tot, rl.q * rl.Wp);
-    const uint32_t pos0 = before_row + seg_excl_scan(c0, rl.Wp, rl.w);
-    const uint32_t pos1 = before_row + X + seg_excl_scan(c1, rl.Wp, rl.w);
+    const uint32_t pos0 = before_row + (ex01 & 0xFFFFu);
+    const uint32_t pos1 = before_row + X + (ex01 >> 16);
     const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     // Two lists, x-edges and then y/z-edges, so that the lanes of one step take the same load
@@ -590,9 +596,14 @@ __device__ __forceinline__ void emit_rows_triangles(const EmitRows& E, EmitRowsL
             combo_bases(counts, voff, rows, cb);
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < 8; q += 2) {
+            // two combinations per segmented scan (row totals below 2^16)
+            const uint32_t ex = seg_excl_scan((uint32_t)__popcll(cw[q]) | (uint32_t)__popcll(cw[q + 1]) << 16, rl.Wp,
+                                              rl.w);
             S.cut[q][lane] = cw[q];
-            S.base[q][lane] = (uint32_t)(cb[q] - v0) + seg_excl_scan((uint32_t)__popcll(cw[q]), rl.Wp, rl.w);
+            S.cut[q + 1][lane] = cw[q + 1];
+            S.base[q][lane] = (uint32_t)(cb[q] - v0) + (ex & 0xFFFFu);
+            S.base[q + 1][lane] = (uint32_t)(cb[q + 1] - v0) + (ex >> 16);
         }
     }
     uint32_t lo[8], hi[8];

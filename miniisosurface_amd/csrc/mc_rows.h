@@ -174,9 +174,13 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
             combo_bases(counts, voff, rows, cb);
         }
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < 8; q += 2) {
+            const uint32_t ex = seg_excl_scan((uint32_t)__popcll(cw[q]) | (uint32_t)__popcll(cw[q + 1]) << 16, rl.Wp,
+                                              rl.w);
             S.cut[q][lane] = cw[q];
-            S.base[q][lane] = (uint32_t)(cb[q] - v0) + seg_excl_scan((uint32_t)__popcll(cw[q]), rl.Wp, rl.w);
+            S.cut[q + 1][lane] = cw[q + 1];
+            S.base[q][lane] = (uint32_t)(cb[q] - v0) + (ex & 0xFFFFu);
+            S.base[q + 1][lane] = (uint32_t)(cb[q + 1] - v0) + (ex >> 16);
         }
     }
     uint32_t lo[8], hi[8];
