@@ -288,7 +288,7 @@ def main():
         summarize_pmc(*args.summarize_pmc, command=args.pmc_command)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py measures one GPU: the engine has no multi-GPU path (volumes would shard by z-slabs)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 """Builds libminiiso.so (HIP kernels + C ABI) for gfx950 in-tree.
 
-    python -m miniisosurface_amd.build        (or __graft_entry__.build())
+    python -m miniisosurface_amd.build
 
 Flags that matter for parity with the reference:
   -ffp-contract=off                     a + w*(b - a) stays a separate multiply and add

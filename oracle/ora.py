@@ -1,7 +1,8 @@
 """ctypes wrapper over oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
 
-Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module, and only
-as the checker (or the reported CPU baseline). The product never loads it.
+Only tests/, the smoke check of the repository's entry module and bench.py's CPU-baseline leg
+import this module, and only as the checker (or the reported CPU baseline). The product never
+loads it.
 """
 from __future__ import annotations
 

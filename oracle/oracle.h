@@ -2,8 +2,8 @@
  * oracle.h -- CPU restatement of the reference isosurface hot path.
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product (miniisosurface_amd/, include/) links,
- * loads or calls this code. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
- * leg use it, and there only as the checker / reported CPU baseline.
+ * loads or calls this code. Only tests/, the smoke check of the repository's entry module and
+ * bench.py's CPU-baseline leg use it, and there only as the checker / reported CPU baseline.
  *
  * What it restates (reference paths relative to the miniIsosurface root):
  *   ora_fe_*  : flyingEdges/serial/FlyingEdgesAlgorithm.cpp (pass1 :23-66, pass2 :72-190,
