@@ -115,7 +115,8 @@ def read_traffic(kernel):
 
 def bench_c2(reps=10):
     """512^3 gyroid (8 periods) + U(-0.05, 0.05) noise (seed 0), iso 0: Flying Edges and Marching
-    Cubes, volume resident in HBM, one complete extraction per step."""
+    Cubes (first-visit order, and edge order as the duplicate-free variant numbers vertices),
+    volume resident in HBM, one complete extraction per step."""
     import torch
 
     import miniisosurface_amd as m
@@ -126,7 +127,8 @@ def bench_c2(reps=10):
     n = 512
     cells = (n - 1) ** 3
     out = {}
-    for name, algo in (("flying_edges", m.FlyingEdgesAlgorithm(dv, 0.0)), ("marching_cubes", m._MC(dv, 0.0))):
+    for name, algo in (("flying_edges", m.FlyingEdgesAlgorithm(dv, 0.0)), ("marching_cubes", m._MC(dv, 0.0)),
+                       ("marching_cubes_edge_order", m._MC(dv, 0.0, order="edge"))):
         for _ in range(3):
             algo.run()
         torch.cuda.synchronize()

@@ -112,6 +112,16 @@ int miso_mc_device_output(miso_mc* h, const float** points, const float** normal
                           const void** triangles, int* index_bits);
 int miso_mc_time_ms(const miso_mc* h, float* ms);
 void miso_mc_destroy(miso_mc* h);
+/* Vertex order of the next runs. FIRST_VISIT (the default): the serial reference's
+ * (marchingCubes/serial/main.cpp:145-177). EDGE: sorted by global edge id, the order of the
+ * duplicate-free OpenMP variant (util::duplicateRemover, marchingCubes/util/DuplicateRemover.h:
+ * 34-70, called at marchingCubes/openmpDupFree/main.cpp:324-331); deterministic, with triangles
+ * in cell order. It equals that variant's output for one section, and for any sectioning when
+ * spacing is 1 and origin 0; otherwise a seam point there may come from a later section's cell
+ * and differ in the last place. */
+#define MISO_MC_ORDER_FIRST_VISIT 0
+#define MISO_MC_ORDER_EDGE 1
+int miso_mc_set_order(miso_mc* h, int order);
 
 /* ---------------------------------------------------------------- batched Flying Edges */
 /* No counterpart in the reference, which extracts one volume per process

@@ -6,6 +6,8 @@
  *       flyingEdges/serial/FlyingEdgesAlgorithm.h:27-49 (OpenMP: flyingEdges/openmp/...h:29-51)
  *   MarchingCubes(image, isoval) / MarchingCubes(image, isoval, nSectionsX, nSectionsY, nSectionsZ)
  *       marchingCubes/serial/main.cpp:42-44, marchingCubes/openmp/main.cpp:159-162
+ *   MarchingCubesDupFree(image, isoval, nSectionsX, nSectionsY, nSectionsZ)
+ *       marchingCubes/openmpDupFree/main.cpp:159-162 (the same signature, edge-sorted vertices)
  *
  * The image and mesh types are template parameters, so this header needs none of the miniapp's
  * headers: Image must provide pointer() (the float data, x fastest), x/y/zdimension(),
@@ -106,7 +108,7 @@ private:
 };
 
 template <class Mesh, class Image, class T>
-Mesh MarchingCubes(Image const& image, T const& isoval)
+Mesh MarchingCubes(Image const& image, T const& isoval, int order = MISO_MC_ORDER_FIRST_VISIT)
 {
     const miso_volume v = host_volume(image);
     miso_mc* h = nullptr;
@@ -115,6 +117,7 @@ Mesh MarchingCubes(Image const& image, T const& isoval)
         miso_mc* h;
         ~Guard() { miso_mc_destroy(h); }
     } guard{h};
+    if (order != MISO_MC_ORDER_FIRST_VISIT) check(miso_mc_set_order(h, order), "MarchingCubes order");
     check(miso_mc_run(h), "MarchingCubes run");
     return collect<Mesh>([&](uint64_t* a, uint64_t* b) { return miso_mc_counts(h, a, b); },
                          [&](float* p, float* n, uint64_t* t) { return miso_mc_copy_out(h, p, n, t); });
@@ -125,6 +128,14 @@ template <class Mesh, class Image, class T>
 Mesh MarchingCubes(Image const& image, T const& isoval, unsigned, unsigned, unsigned)
 {
     return MarchingCubes<Mesh>(image, isoval);
+}
+
+/* the duplicate-free OpenMP variant (marchingCubes/openmpDupFree/main.cpp:159-331): its mesh,
+ * vertices sorted by global edge id as util::duplicateRemover leaves them */
+template <class Mesh, class Image, class T>
+Mesh MarchingCubesDupFree(Image const& image, T const& isoval, unsigned, unsigned, unsigned)
+{
+    return MarchingCubes<Mesh>(image, isoval, MISO_MC_ORDER_EDGE);
 }
 
 }  // namespace miniiso

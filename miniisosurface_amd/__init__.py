@@ -184,8 +184,17 @@ def flying_edges(image, isoval, stream=None) -> TriangleMesh:
     return algo.moveOutput()
 
 
+MC_ORDERS = {"first_visit": 0, "edge": 1}
+
+
 class _MC:
-    def __init__(self, image, isoval, stream=None):
+    """order "first_visit": the serial reference's vertex numbering; "edge": vertices sorted by
+    global edge id, as util::duplicateRemover leaves them (marchingCubes/util/DuplicateRemover.h:34-70)."""
+
+    def __init__(self, image, isoval, stream=None, order="first_visit"):
+        if order not in MC_ORDERS:
+            raise ValueError(f"order must be one of {sorted(MC_ORDERS)}")
+        self._h = None
         self._vol, self._keep = _volume_struct(image)
         h = ctypes.c_void_p()
         check(lib().miso_mc_create(ctypes.byref(self._vol), ctypes.c_float(float(np.float32(isoval))),
@@ -193,6 +202,14 @@ class _MC:
               "MarchingCubes")
         self._h = h
         self._destroy = lib().miso_mc_destroy
+        if MC_ORDERS[order]:
+            self.set_order(order)
+
+    def set_order(self, order):
+        """Vertex order of the next run()s (miso_mc_set_order)."""
+        if order not in MC_ORDERS:
+            raise ValueError(f"order must be one of {sorted(MC_ORDERS)}")
+        check(lib().miso_mc_set_order(self._h, MC_ORDERS[order]), "MarchingCubes")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -222,11 +239,13 @@ class _MC:
         return TriangleMesh(pts, nrm, tris)
 
 
-def MarchingCubes(image, isoval, nSectionsX=None, nSectionsY=None, nSectionsZ=None, stream=None) -> TriangleMesh:
+def MarchingCubes(image, isoval, nSectionsX=None, nSectionsY=None, nSectionsZ=None, stream=None,
+                  order="first_visit") -> TriangleMesh:
     """MarchingCubes(image, isoval) (marchingCubes/serial/main.cpp:42-192). The OpenMP overload's
     section counts (marchingCubes/openmp/main.cpp:159-162) are accepted and ignored: the result is
     the serial numbering, which the OpenMP variant's output equals up to vertex order and seam
-    duplicates."""
-    mc = _MC(image, isoval, stream)
+    duplicates. order="edge" gives the duplicate-free OpenMP variant's vertex order instead
+    (marchingCubes/openmpDupFree/main.cpp:324-331)."""
+    mc = _MC(image, isoval, stream, order)
     mc.run()
     return mc.mesh()

@@ -71,6 +71,7 @@ def lib():
         "miso_mc_copy_out": [P, P, P, P],
         "miso_mc_device_output": [P, PP, PP, PP, ctypes.POINTER(I)],
         "miso_mc_time_ms": [P, ctypes.POINTER(F)],
+        "miso_mc_set_order": [P, I],
         "miso_batch_create": [ctypes.POINTER(Volume), ctypes.POINTER(F), ctypes.c_int64, P, PP],
         "miso_batch_run": [P],
         "miso_batch_totals": [P, U64P, U64P],

@@ -1,8 +1,13 @@
 /*
  * marching_cubes.cpp -- command-line driver for Marching Cubes on the GPU, with the miniapp's
  * flags, report and output (marchingCubes/serial/main.cpp:194-330), including -input_dat.
+ * -dup_free writes the duplicate-free OpenMP variant's vertex order instead (vertices sorted by
+ * global edge id, marchingCubes/openmpDupFree/main.cpp:324-331); that driver's section flags
+ * (-sx, -sy, -sz, marchingCubes/openmpDupFree/main.cpp:376-387) are accepted and do not change
+ * the result.
  *
  *   miniiso_marching_cubes -i volume.vtk | -input_dat volume.dat -o mesh.vtk -v isoval [-y report.yaml]
+ *                          [-dup_free] [-sx N -sy N -sz N]
  */
 #include <cstdlib>
 #include <cstring>
@@ -19,7 +24,7 @@ using namespace miso_apps;
 int main(int argc, char* argv[])
 {
     float isoval = 0.0f;
-    bool isoval_set = false, use_dat = false;
+    bool isoval_set = false, use_dat = false, dup_free = false;
     const char* in = nullptr;
     const char* out = nullptr;
     std::string yaml;
@@ -37,6 +42,12 @@ int main(int argc, char* argv[])
             isoval = (float)std::atof(argv[++i]);
         } else if ((!strcmp(argv[i], "-y") || !strcmp(argv[i], "-yaml_output_file")) && more) {
             yaml = argv[++i];
+        } else if (!strcmp(argv[i], "-dup_free")) {
+            dup_free = true;
+        } else if ((!strcmp(argv[i], "-sx") || !strcmp(argv[i], "-sections_x") || !strcmp(argv[i], "-sy") ||
+                    !strcmp(argv[i], "-sections_y") || !strcmp(argv[i], "-sz") || !strcmp(argv[i], "-sections_z")) &&
+                   more) {
+            ++i;
         } else if (!strcmp(argv[i], "-h") || !strcmp(argv[i], "-help")) {
             std::cout << "Marching Cubes (MI355X) Options:\n"
                          "  -input_file (-i)\n"
@@ -44,6 +55,8 @@ int main(int argc, char* argv[])
                          "  -output_file (-o)\n"
                          "  -isoval (-v)\n"
                          "  -yaml_output_file (-y)\n"
+                         "  -dup_free (vertices sorted by global edge id)\n"
+                         "  -sections_x (-sx), -sections_y (-sy), -sections_z (-sz) (accepted, no effect)\n"
                          "  -help (-h)\n";
             return 0;
         }
@@ -63,7 +76,8 @@ int main(int argc, char* argv[])
         doc.add("File y-dimension", image.ydimension());
         doc.add("File z-dimension", image.zdimension());
         RunTimer run;
-        Mesh mesh = miniiso::MarchingCubes<Mesh>(image, isoval);
+        Mesh mesh = miniiso::MarchingCubes<Mesh>(image, isoval,
+                                                 dup_free ? MISO_MC_ORDER_EDGE : MISO_MC_ORDER_FIRST_VISIT);
         run.stop();
         doc.add("Number of vertices in mesh", mesh.numberOfVertices());
         doc.add("Number of triangles in mesh", mesh.numberOfTriangles());

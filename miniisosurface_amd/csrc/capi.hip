@@ -235,6 +235,9 @@ struct McState {
     int64_t ntiles = 0;
     void* d_map = nullptr;           // Flying Edges vertex id -> Marching Cubes vertex id
     uint64_t map_bytes = 0;
+    int order = MISO_MC_ORDER_FIRST_VISIT;
+    uint64_t* d_xyz = nullptr;       // edge order: per-row exclusive scans of the x, y, z counts
+    uint64_t* d_axstate = nullptr;   // their three scan states, then the three totals
 
     int init(const Geom& g)
     {
@@ -248,18 +251,30 @@ struct McState {
         return MISO_OK;
     }
 
+    /* the edge-order buffers, allocated on first use */
+    int init_edge_order(const Geom& g)
+    {
+        if (d_xyz) return MISO_OK;
+        const int64_t R = g.ny * g.nz;
+        if (int rc = dev_alloc(&d_xyz, (size_t)(3 * R))) return rc;
+        return dev_alloc(&d_axstate, (size_t)(3 * (1 + ntiles) + 3));
+    }
+
     void release()
     {
         dev_free(d_owned);
         dev_free(d_moff);
         dev_free(d_scan);
         dev_free(d_total);
+        dev_free(d_xyz);
+        dev_free(d_axstate);
         if (d_map) (void)hipFree(d_map);
         d_map = nullptr;
     }
 
     int launch_count(RowState& s)
     {
+        if (order == MISO_MC_ORDER_EDGE) return MISO_OK;   // no first-visit offsets needed
         if (s.lane_rows())
             hipLaunchKernelGGL(k_mc_count_rows, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
                                (const uint64_t*)s.d_mask, s.g, d_owned, d_scan, (int64_t)(1 + ntiles));
@@ -284,6 +299,48 @@ struct McState {
             if (int rc = dev_alloc(&p, (size_t)need)) return rc;
             d_map = p;
             map_bytes = need;
+        }
+        if (order == MISO_MC_ORDER_EDGE) {
+            // vertices sorted by global edge id: the permutation from the three axis scans,
+            // then owners write at perm[FE id]; triangles through the same permutation
+            const int64_t R = s.g.ny * s.g.nz;
+            const int64_t sw = 1 + ntiles;
+            hipLaunchKernelGGL(k_batch_zero, dim3(64), dim3(256), 0, s.stream, d_axstate, 3 * sw + 3);
+            for (int a = 0; a < 3; ++a)
+                hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)ntiles), dim3(256), 0, s.stream,
+                                   (const uint32_t*)s.d_counts + a, R, d_xyz + a * R, d_axstate + a * sw, ntiles,
+                                   d_axstate + 3 * sw + a, (int64_t)4);
+            hipLaunchKernelGGL(k_edge_order_perm<Index>, dim3((unsigned)((R + 3) / 4)), dim3(256), 0, s.stream,
+                               (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff, (const uint64_t*)d_xyz,
+                               (const uint64_t*)(d_axstate + 3 * sw), R, (Index*)d_map);
+            const uint64_t* no_moff = nullptr;
+            if (!s.lane_rows()) {
+                hipLaunchKernelGGL((k_mc_vertices<Index, true>), dim3(s.row_blocks()), dim3(256), 0, s.stream,
+                                   s.d_vol, (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
+                                   (const uint64_t*)s.d_voff, no_moff, pts, nrm, (Index*)d_map);
+                HIPCALL(hipGetLastError());
+                hipLaunchKernelGGL(k_mc_triangles<Index>, dim3(s.row_blocks()), dim3(256), 0, s.stream,
+                                   (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,
+                                   (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (const Index*)d_map, tris);
+                HIPCALL(hipGetLastError());
+                return MISO_OK;
+            }
+            if (s.unit_geometry())
+                hipLaunchKernelGGL((k_mc_vertices_rows<Index, true, true>), dim3(s.lane_row_blocks()), dim3(256), 0,
+                                   s.stream, s.d_vol, (const uint64_t*)s.d_mask, s.g, iso,
+                                   (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff, no_moff, pts, nrm,
+                                   (Index*)d_map);
+            else
+                hipLaunchKernelGGL((k_mc_vertices_rows<Index, false, true>), dim3(s.lane_row_blocks()), dim3(256), 0,
+                                   s.stream, s.d_vol, (const uint64_t*)s.d_mask, s.g, iso,
+                                   (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff, no_moff, pts, nrm,
+                                   (Index*)d_map);
+            HIPCALL(hipGetLastError());
+            hipLaunchKernelGGL(k_mc_triangles_rows<Index>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                               (const uint64_t*)s.d_mask, s.g, (const RowCounts*)s.d_counts,
+                               (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (const Index*)d_map, tris);
+            HIPCALL(hipGetLastError());
+            return MISO_OK;
         }
         if (s.lane_rows()) {
             if (s.unit_geometry())
@@ -718,6 +775,18 @@ extern "C" int miso_mc_run(miso_mc* h)
     HIPCALL(hipStreamSynchronize(s.stream));
     HIPCALL(hipEventElapsedTime(&h->ms, h->ev[0], h->ev[1]));
     h->done = true;
+    return MISO_OK;
+}
+
+extern "C" int miso_mc_set_order(miso_mc* h, int order)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (order != MISO_MC_ORDER_FIRST_VISIT && order != MISO_MC_ORDER_EDGE)
+        return fail(MISO_ERR_ARG, "order must be MISO_MC_ORDER_FIRST_VISIT or MISO_MC_ORDER_EDGE");
+    if (order == MISO_MC_ORDER_EDGE && has_cells(h->st.g))
+        if (int rc = h->mc.init_edge_order(h->st.g)) return rc;
+    h->mc.order = order;
+    h->done = false;
     return MISO_OK;
 }
 

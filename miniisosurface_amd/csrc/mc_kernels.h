@@ -85,10 +85,11 @@ __global__ __launch_bounds__(256) void k_mc_count(const uint64_t* __restrict__ m
     if (lane == 0) owned[r] = total;
 }
 
-/* Exclusive scan of a uint32 per grid row into uint64 offsets (same look-back as pass 3). */
+/* Exclusive scan of a uint32 per grid row (vals[r * stride]) into uint64 offsets (same look-back
+ * as pass 3). */
 __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* __restrict__ vals, int64_t R,
                                                   uint64_t* __restrict__ out, uint64_t* __restrict__ state,
-                                                  int64_t ntiles, uint64_t* __restrict__ total)
+                                                  int64_t ntiles, uint64_t* __restrict__ total, int64_t stride = 1)
 {
     __shared__ int64_t s_tile;
     __shared__ uint64_t s_w[4];
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* __restrict__ v
 #pragma unroll
     for (int u = 0; u < kScanItems; ++u) {
         const int64_t r = r0 + u;
-        v[u] = r < R ? vals[r] : 0;
+        v[u] = r < R ? vals[r * stride] : 0;
         sv += v[u];
     }
     const uint64_t iv = wave_incl_scan_u64(sv);
@@ -141,7 +142,9 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t* __restrict__ v
     }
 }
 
-template <typename Index>
+/* EdgeOrder: each owned edge's point goes to fe_to_mc[its Flying Edges id] (an edge-order
+ * permutation made beforehand), which this kernel then only reads; moff is not used. */
+template <typename Index, bool EdgeOrder = false>
 __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ vol,
                                                      const uint64_t* __restrict__ mask, Geom g, float iso,
                                                      const RowCounts* __restrict__ counts,
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ v
     EmitLds& L = lds_all[wave];
     uint64_t cbase[8];
     combo_bases(counts, voff, rw.rows, cbase);
-    uint64_t mrun = moff[r];
+    uint64_t mrun = EdgeOrder ? 0 : moff[r];
     const float Y = g.org[1] + (float)j * g.sp[1], Z = g.org[2] + (float)k * g.sp[2];
     for (int64_t s0w = 0; s0w < g.W; s0w += kSegWords) {
         rw.stage(s0w);
@@ -200,13 +203,15 @@ __global__ __launch_bounds__(256) void k_mc_vertices(const float* __restrict__ v
                     B[a] = A[a] + g.sp[a];
                     float p[3], nn[3];
                     edge_point(vol, g, iso, a, i + dx, j + dy, k + dz, A, B, p, nn);
+                    const uint64_t fe = rw.fe_vertex_id(cbase, ed, li);
+                    if (EdgeOrder) vid = (uint64_t)fe_to_mc[fe];
                     pts[3 * vid + 0] = p[0];
                     pts[3 * vid + 1] = p[1];
                     pts[3 * vid + 2] = p[2];
                     nrm[3 * vid + 0] = nn[0];
                     nrm[3 * vid + 1] = nn[1];
                     nrm[3 * vid + 2] = nn[2];
-                    fe_to_mc[rw.fe_vertex_id(cbase, ed, li)] = (Index)vid;
+                    if (!EdgeOrder) fe_to_mc[fe] = (Index)vid;
                     ++vid;
                 }
             }

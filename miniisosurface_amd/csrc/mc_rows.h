@@ -127,7 +127,35 @@ __device__ __forceinline__ void mc_corners(const Geom& g, int64_t i, int64_t j, 
     B[a] = A[a] + g.sp[a];
 }
 
-template <typename Index, bool Unit>
+/* Edge order (the order of util::duplicateRemover, marchingCubes/util/DuplicateRemover.h:34-70:
+ * vertices sorted by global edge id, marchingCubes/util/Image3D.cpp:170-253): all cut x-edges in
+ * (k, j, i) order, then all y-edges, then all z-edges. perm maps each Flying Edges vertex id
+ * (row blocks [x][y][z]) to that order; xyz holds the per-row exclusive scans of the x, y and z
+ * counts (R each) and tot their totals. One wavefront per grid row. */
+template <typename Index>
+__global__ __launch_bounds__(256) void k_edge_order_perm(const RowCounts* __restrict__ counts,
+                                                         const uint64_t* __restrict__ voff,
+                                                         const uint64_t* __restrict__ xyz,
+                                                         const uint64_t* __restrict__ tot, int64_t R,
+                                                         Index* __restrict__ perm)
+{
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= R) return;
+    const int lane = lane_id();
+    const RowCounts c = counts[r];
+    const uint64_t base[3] = {xyz[r], tot[0] + xyz[R + r], tot[0] + tot[1] + xyz[2 * R + r]};
+    const uint32_t n[3] = {c.x, c.y, c.z};
+    uint64_t f = voff[r];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (uint32_t t = lane; t < n[a]; t += 64) perm[f + t] = (Index)(base[a] + t);
+        f += n[a];
+    }
+}
+
+/* EdgeOrder: each owned edge's point goes to fe_to_mc[its Flying Edges id] (an edge-order
+ * permutation made beforehand), which this kernel then only reads. */
+template <typename Index, bool Unit, bool EdgeOrder = false>
 __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restrict__ vol,
                                                           const uint64_t* __restrict__ mask, Geom g, float iso,
                                                           const RowCounts* __restrict__ counts,
@@ -195,7 +223,7 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
     const uint32_t chi[8] = {hi[0], hi[1], hi[3], hi[2], hi[4], hi[5], hi[7], hi[6]};
     const uint32_t cfirst = cincl - ncell;
     const int64_t j0 = r0 % g.ny, k0 = r0 / g.ny;
-    uint64_t mid = moff[r0];   // the wavefront's first Marching Cubes vertex
+    uint64_t mid = EdgeOrder ? 0 : moff[r0];   // the wavefront's first Marching Cubes vertex
     for (uint32_t win = 0; win < ctotal; win += kMcCellCap) {
         const uint32_t wend = win + kMcCellCap;
         if (ncell && cfirst < wend && cfirst + ncell > win) {
@@ -276,18 +304,19 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
                     edge_point(vol, g, iso, edge_axis(e), i + edge_dx(e), jj + edge_dy(e), kk + edge_dz(e), A, B, pp,
                                nn);
                 }
-                const uint64_t id = mid + v;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    pts[3 * id + c] = pp[c];
-                    nrm[3 * id + c] = nn[c];
-                }
                 // the edge's Flying Edges id, as the triangle pass computes it
                 const int cq = combo_of_edge(e);
                 const int p = b + edge_dx(e);
                 const uint64_t cwq = S.cut[cq][src];
                 const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
-                fe_to_mc[v0 + S.base[cq][src] + (uint32_t)__popcll(below)] = (Index)id;
+                const uint64_t fe = v0 + S.base[cq][src] + (uint32_t)__popcll(below);
+                const uint64_t id = EdgeOrder ? (uint64_t)fe_to_mc[fe] : mid + v;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    pts[3 * id + c] = pp[c];
+                    nrm[3 * id + c] = nn[c];
+                }
+                if (!EdgeOrder) fe_to_mc[fe] = (Index)id;
             }
             mid += vtot;
             wave_sync();

@@ -17,7 +17,8 @@ struct ora_mc {
     float* pts;
     float* nrm;
     uint64_t* tris;
-    size_t npts, ntris, cap_pts, cap_tris;
+    uint64_t* gid;      /* global edge id of each point (Image3D.cpp:168-256) */
+    size_t npts, ntris, cap_pts, cap_tris, cap_gid;
     double seconds;
 };
 
@@ -163,6 +164,11 @@ ora_mc* ora_mc_run(const float* d, int64_t nx_, int64_t ny_, int64_t nz_,
                     }
                     push3(&h->pts, h->npts, &cap_pts, p);
                     push3(&h->nrm, h->npts, &cap_nrm, n);
+                    if (h->npts + 1 > h->cap_gid) {
+                        h->cap_gid = h->cap_gid ? h->cap_gid * 2 : 1024;
+                        h->gid = realloc(h->gid, h->cap_gid * sizeof(uint64_t));
+                    }
+                    h->gid[h->npts] = ge;
                     h->npts++;
                 }
                 tri[s] = id;
@@ -197,9 +203,14 @@ void ora_mc_mesh(const ora_mc* h, float* pts, float* nrm, uint64_t* tris)
     if (tris && h->ntris) memcpy(tris, h->tris, h->ntris * 3 * sizeof(uint64_t));
 }
 
+void ora_mc_edge_ids(const ora_mc* h, uint64_t* gid)
+{
+    if (gid && h->npts) memcpy(gid, h->gid, h->npts * sizeof(uint64_t));
+}
+
 void ora_mc_free(ora_mc* h)
 {
     if (!h) return;
-    free(h->pts); free(h->nrm); free(h->tris);
+    free(h->pts); free(h->nrm); free(h->tris); free(h->gid);
     free(h);
 }

@@ -46,6 +46,7 @@ def lib():
         L.ora_mc_run.argtypes = [_P, _I64, _I64, _I64, _P, _P, ctypes.c_float]
         L.ora_mc_sizes.argtypes = [_P, _P, _P, _P]
         L.ora_mc_mesh.argtypes = [_P, _P, _P, _P]
+        L.ora_mc_edge_ids.argtypes = [_P, _P]
         L.ora_mc_free.argtypes = [_P]
         _lib = L
     return _lib
@@ -112,8 +113,22 @@ class MCResult:
             self.normals = np.zeros((self.nverts, 3), np.float32)
             self.triangles = np.zeros((self.ntris, 3), np.uint64)
             L.ora_mc_mesh(h, _ptr(self.points), _ptr(self.normals), _ptr(self.triangles))
+            self.edge_ids = np.zeros(self.nverts, np.uint64)
+            L.ora_mc_edge_ids(h, _ptr(self.edge_ids))
         finally:
             L.ora_mc_free(h)
+
+    def edge_order(self):
+        """(points, normals, triangles) renumbered as util::duplicateRemover renumbers them
+        (marchingCubes/util/DuplicateRemover.h:34-70) when the duplicate-free OpenMP driver runs
+        one section (marchingCubes/openmpDupFree/main.cpp:198-331): points sorted by global edge
+        id, which is unique per point here, and each triangle's ids sent through the old -> new
+        map; triangles keep their cell order."""
+        order = np.argsort(self.edge_ids, kind="stable")
+        old_to_new = np.empty(self.nverts, np.uint64)
+        old_to_new[order] = np.arange(self.nverts, dtype=np.uint64)
+        tris = old_to_new[self.triangles.astype(np.int64)] if self.ntris else self.triangles.copy()
+        return self.points[order], self.normals[order], tris
 
 
 def ref_binary(name: str) -> str | None:

@@ -13,7 +13,9 @@
  *               Geometry helpers: flyingEdges/util/Image3D.cpp:20-188.
  *   ora_mc_*  : marchingCubes/serial/main.cpp:42-192 (first-visit vertex numbering through a
  *               map from global edge id, marchingCubes/util/Image3D.cpp:168-256) with
- *               findCaseId / interpolate from marchingCubes/util/util.h:24-54.
+ *               findCaseId / interpolate from marchingCubes/util/util.h:24-54. The edge ids it
+ *               records per point give, in ora.py, the duplicate-free OpenMP variant's order
+ *               (marchingCubes/util/DuplicateRemover.h:34-70) for one section.
  *
  * Pinning: every function here is checked in tests/ against fixtures written by the compiled
  * reference (tests/golden/), against oracle/_ref where it is built, and against closed forms
@@ -63,6 +65,8 @@ ora_mc* ora_mc_run(const float* data, int64_t nx, int64_t ny, int64_t nz,
                    const float spacing[3], const float origin[3], float iso);
 void    ora_mc_sizes(const ora_mc* h, uint64_t* npts, uint64_t* ntris, double* seconds);
 void    ora_mc_mesh(const ora_mc* h, float* pts, float* nrm, uint64_t* tris);
+/* global edge id (marchingCubes/util/Image3D.cpp:168-256) of each point, in point order */
+void    ora_mc_edge_ids(const ora_mc* h, uint64_t* gid);
 void    ora_mc_free(ora_mc* h);
 
 #ifdef __cplusplus

@@ -2,7 +2,8 @@
 
 The binaries are compiled from the reference sources by `make -C oracle ref` (outputs in
 oracle/_ref/, never committed). For each fixture this script writes the input volume as a VTK
-file, runs flyingEdgesSerial and the serial Marching Cubes (`mcSerial`) on it with the
+file, runs flyingEdgesSerial, the serial Marching Cubes (`mcSerial`) and the duplicate-free
+OpenMP Marching Cubes (`mcOpenMPDupFree`, one section: -sz 1, one thread) on it with the
 reference's own command line (-i, -o, -v, -y), reads the meshes they save, and stores inputs and
 outputs in one .npz per fixture. Nothing from the reference is stored except these outputs.
 
@@ -25,15 +26,17 @@ from miniisosurface_amd import synth, vtkio  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 REF = os.path.join(ROOT, "oracle", "_ref")
+DUPFREE_ONE_SECTION = ("-sx", "1", "-sy", "1", "-sz", "1")
 
 
-def run_ref(binary, vol, iso, spacing, origin, tmp):
+def run_ref(binary, vol, iso, spacing, origin, tmp, extra=()):
     inp = os.path.join(tmp, "in.vtk")
     out = os.path.join(tmp, "out.vtk")
     vtkio.save_image(inp, vtkio.Image3D(vol, spacing, origin))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
     subprocess.run([os.path.join(REF, binary), "-i", inp, "-o", out, "-v", repr(float(iso)),
-                    "-y", os.path.join(tmp, "y", "r.yaml")],
-                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tmp)
+                    "-y", os.path.join(tmp, "y", "r.yaml"), *extra],
+                   check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, cwd=tmp, env=env)
     m = vtkio.load_mesh(out)
     return m.points, m.normals, m.triangles
 
@@ -43,11 +46,13 @@ def fixture(name, vol, iso, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0), sto
     with tempfile.TemporaryDirectory() as tmp:
         fp, fn, ft = run_ref("flyingEdgesSerial", vol, iso, spacing, origin, tmp)
         mp, mn, mt = run_ref("mcSerial", vol, iso, spacing, origin, tmp)
+        dp, dn, dt = run_ref("mcOpenMPDupFree", vol, iso, spacing, origin, tmp, DUPFREE_ONE_SECTION)
     d = dict(iso=np.float32(iso), spacing=np.array(spacing, np.float32),
              origin=np.array(origin, np.float32), dims=np.array(vol.shape[::-1], np.int64),
              input_sha256=np.array(hashlib.sha256(vol.tobytes()).hexdigest()),
              fe_points=fp, fe_normals=fn, fe_triangles=ft.astype(np.uint64),
-             mc_points=mp, mc_normals=mn, mc_triangles=mt.astype(np.uint64))
+             mc_points=mp, mc_normals=mn, mc_triangles=mt.astype(np.uint64),
+             mcdf_points=dp, mcdf_normals=dn, mcdf_triangles=dt.astype(np.uint64))
     if store_input:
         d["volume"] = vol
     np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)
@@ -59,7 +64,7 @@ def cases256():
     """One 2x2x2 volume per marching-cubes case: corner q is above iso iff bit q of the case."""
     rng = np.random.default_rng(256)
     corner_ijk = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
-    vols, fe, mc = [], [], []
+    vols, fe, mc, df = [], [], [], []
     with tempfile.TemporaryDirectory() as tmp:
         for c in range(256):
             v = np.zeros((2, 2, 2), np.float32)
@@ -69,6 +74,7 @@ def cases256():
             vols.append(v)
             fe.append(run_ref("flyingEdgesSerial", v, 0.5, (1, 1, 1), (0, 0, 0), tmp))
             mc.append(run_ref("mcSerial", v, 0.5, (1, 1, 1), (0, 0, 0), tmp))
+            df.append(run_ref("mcOpenMPDupFree", v, 0.5, (1, 1, 1), (0, 0, 0), tmp, DUPFREE_ONE_SECTION))
 
     def pack(res, prefix):
         out = {}
@@ -80,7 +86,7 @@ def cases256():
         return out
 
     np.savez_compressed(os.path.join(OUT, "cases256.npz"), volumes=np.stack(vols), iso=np.float32(0.5),
-                        **pack(fe, "fe_"), **pack(mc, "mc_"))
+                        **pack(fe, "fe_"), **pack(mc, "mc_"), **pack(df, "mcdf_"))
     print("cases256 written")
 
 

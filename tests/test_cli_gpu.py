@@ -76,6 +76,32 @@ def test_marching_cubes_cli_matches_reference_driver(oracle_mod, name):
         assert "Mini-Application Name: Marching Cubes" in out.stdout
 
 
+@pytest.mark.parametrize("name", FIXTURES[:3])
+def test_marching_cubes_cli_dup_free_matches_reference_driver(oracle_mod, name):
+    """-dup_free against the duplicate-free OpenMP driver, one section (its -sx/-sy/-sz flags
+    passed to both), byte for byte; without that driver, against the oracle's edge order."""
+    from miniisosurface_amd import Image3D, TriangleMesh, save_image, save_mesh
+    p = [g for g in golden_files() if name in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    sections = ["-sx", "1", "-sy", "1", "-sz", "1"]
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        out = subprocess.run([_tool("miniiso_marching_cubes"), "-i", "v.vtk", "-o", "ours.vtk", "-v",
+                              repr(float(iso)), "-dup_free", *sections], cwd=tmp, capture_output=True, text=True,
+                             timeout=300)
+        assert out.returncode == 0, out.stderr
+        ours = open(os.path.join(tmp, "ours.vtk"), "rb").read()
+        ref = oracle_mod.ref_binary("mcOpenMPDupFree")
+        if ref:
+            subprocess.run([ref, "-i", "v.vtk", "-o", "expected.vtk", "-v", repr(float(iso)), *sections],
+                           cwd=tmp, check=True, capture_output=True, timeout=300,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"))
+        else:
+            save_mesh(os.path.join(tmp, "expected.vtk"),
+                      TriangleMesh(*oracle_mod.MCResult(vol, iso, sp, org).edge_order()))
+        assert ours == open(os.path.join(tmp, "expected.vtk"), "rb").read()
+
+
 def test_marching_cubes_cli_dat_input(oracle_mod):
     """-input_dat: 16-bit little-endian dims and samples, unit spacing, zero origin."""
     from miniisosurface_amd import TriangleMesh, load_mesh

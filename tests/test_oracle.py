@@ -178,3 +178,69 @@ def test_reference_binaries_agree_with_oracle(oracle_mod):
             assert bits_equal(r.points, m.points)
             assert bits_equal(r.normals, m.normals)
             assert bits_equal(r.triangles, m.triangles)
+
+
+def _tri_multiset(t):
+    t = np.ascontiguousarray(np.asarray(t, np.uint64).reshape(-1, 3))
+    return np.sort(t.view([("a", np.uint64), ("b", np.uint64), ("c", np.uint64)]).ravel())
+
+
+@pytest.mark.parametrize("path", golden_files(), ids=lambda p: os.path.basename(p)[:-4])
+def test_oracle_mc_edge_order_matches_dupfree_fixture(oracle_mod, path):
+    """The oracle's edge order (restating util::duplicateRemover for one section) against the
+    duplicate-free OpenMP driver's output, one section, stored by tests/golden/gen_fixtures.py."""
+    vol, iso, sp, org, d = load_fixture(path)
+    p, n, t = oracle_mod.MCResult(vol, iso, sp, org).edge_order()
+    assert bits_equal(p, d["mcdf_points"])
+    assert bits_equal(n, d["mcdf_normals"])
+    assert bits_equal(t, d["mcdf_triangles"])
+
+
+def test_oracle_mc_edge_order_all_256_cases(oracle_mod):
+    d = np.load(os.path.join(GOLDEN, "cases256.npz"), allow_pickle=False)
+    vo = np.concatenate([[0], np.cumsum(d["mcdf_vcount"])])
+    to = np.concatenate([[0], np.cumsum(d["mcdf_tcount"])])
+    for c in range(256):
+        p, n, t = oracle_mod.MCResult(d["volumes"][c], 0.5).edge_order()
+        assert bits_equal(p, d["mcdf_points"][vo[c]:vo[c + 1]]), c
+        assert bits_equal(n, d["mcdf_normals"][vo[c]:vo[c + 1]]), c
+        assert bits_equal(t, d["mcdf_triangles"][to[c]:to[c + 1]]), c
+
+
+def test_oracle_mc_edge_order_is_sorted_permutation(oracle_mod):
+    """Properties the order must have whatever the reference: the first-visit mesh's points and
+    triangles, renumbered by a permutation that sorts the global edge ids."""
+    vol = np.random.default_rng(44).uniform(0, 1, (9, 11, 13)).astype(np.float32)
+    r = oracle_mod.MCResult(vol, 0.5)
+    p, n, t = r.edge_order()
+    ids = np.sort(r.edge_ids)
+    assert np.all(np.diff(ids.astype(np.int64)) > 0)              # unique and increasing
+    # each triangle corner's new id is the rank of its edge id among all edge ids
+    old = r.triangles.astype(np.int64)
+    assert np.array_equal(np.searchsorted(ids, r.edge_ids[old]), t.astype(np.int64))
+    # the same triangles, point for point, in the same order
+    assert bits_equal(p[t.astype(np.int64)], r.points[r.triangles.astype(np.int64)])
+    assert bits_equal(n[t.astype(np.int64)], r.normals[r.triangles.astype(np.int64)])
+
+
+def test_reference_dupfree_sections_agree_with_oracle_edge_order(oracle_mod):
+    """Where oracle/_ref is built: the duplicate-free driver with 2 x 3 x 4 sections on 4 threads.
+    With spacing 1 and origin 0 every section computes a seam point bit for bit alike, so points
+    and normals equal the one-section order exactly; triangles come in section order, so they
+    are compared as a multiset of index triples."""
+    exe = oracle_mod.ref_binary("mcOpenMPDupFree")
+    if exe is None:
+        pytest.skip("oracle/_ref not built here")
+    from miniisosurface_amd import synth, vtkio
+    vol = synth.gaussians((37, 29, 23), 5, 12, sigma_range=(3.0, 8.0))
+    with tempfile.TemporaryDirectory() as tmp:
+        vtkio.save_image(os.path.join(tmp, "v.vtk"), vtkio.Image3D(vol))
+        subprocess.run([exe, "-i", "v.vtk", "-o", "m.vtk", "-v", "0.5", "-sx", "2", "-sy", "3", "-sz", "4",
+                        "-y", os.path.join(tmp, "y", "a.yaml")], cwd=tmp, check=True,
+                       stdout=subprocess.DEVNULL, env=dict(os.environ, OMP_NUM_THREADS="4"))
+        m = vtkio.load_mesh(os.path.join(tmp, "m.vtk"))
+    p, n, t = oracle_mod.MCResult(vol, 0.5).edge_order()
+    assert len(p) > 1000
+    assert bits_equal(p, m.points)
+    assert bits_equal(n, m.normals)
+    assert np.array_equal(_tri_multiset(t), _tri_multiset(m.triangles))
