@@ -146,6 +146,55 @@ def bench_c2(reps=10):
     return out
 
 
+def bench_slabs(vol, iso, planes=128, reps=3):
+    """The benchmark volume extracted in z-slabs of `planes` grid planes (FlyingEdgesSlabs): read
+    in place from HBM (the cost of slabbing against the whole-volume run), and streamed from
+    pinned host memory one slab ahead of the computation (PCIe-inclusive; parts stay on the
+    device), and the same with every part copied to host memory."""
+    import numpy as np
+    import torch
+
+    import miniisosurface_amd as m
+
+    nz, ny, nx = vol.shape
+    cells = (nx - 1) * (ny - 1) * (nz - 1)
+
+    def run(image, copy):
+        s = m.FlyingEdgesSlabs(image, iso, planes)
+        n = 0
+        while s.next():
+            if copy:
+                s.copy_part()
+            n += 1
+        p = s.part()
+        return n, p["vbase"] + p["nverts"], p["tbase"] + p["ntris"]
+
+    def timed(image, copy):
+        run(image, copy)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            res = run(image, copy)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, res
+
+    out = {"planes": planes}
+    wall, (nslabs, V, T) = timed(m.DeviceVolume.from_tensor(vol), False)
+    out.update(slabs=nslabs, nverts=V, ntris=T)
+    out["device_volume"] = {"ms_per_step": round(wall * 1e3, 3), "gcells_s": round(cells / wall / 1e9, 2)}
+    host = torch.empty(vol.shape, dtype=torch.float32, pin_memory=True)
+    host.copy_(vol)
+    image = m.Image3D(host.numpy())
+    wall, _ = timed(image, False)
+    out["pinned_host_volume"] = {"ms_per_step": round(wall * 1e3, 3), "gcells_s": round(cells / wall / 1e9, 2),
+                                 "h2d_gb_s": round(4 * vol.numel() / wall / 1e9, 2)}
+    wall, _ = timed(image, True)
+    out["pinned_host_volume_parts_to_host"] = {"ms_per_step": round(wall * 1e3, 3),
+                                               "gcells_s": round(cells / wall / 1e9, 2)}
+    del image, host
+    return out
+
+
 def bench_batched(reps):
     """256 mixed-size Gaussian volumes (dims log-uniform in [32, 256], 1-6 blobs, iso U[0.2, 0.8],
     seed 0), and one 512^3 gyroid at 16 isovalues in [-1, 1]: the batch (one persistent launch per
@@ -279,6 +328,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-slabs", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 1 GiB device copy first (calibrates rocprofv3 byte counters)")
@@ -395,6 +445,8 @@ def main():
         "sparse_iso_0.95": sparse,
         "volume_gen_s": round(gen_s, 2),
     }
+    if not args.no_slabs:
+        out["slabs_128_planes"] = bench_slabs(vol, args.iso)
     if not args.no_c2:
         out["c2_gyroid_512_iso0"] = bench_c2()
     if not args.no_batch:

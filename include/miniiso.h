@@ -99,6 +99,33 @@ int miso_fe_cube_cases(miso_fe* h, uint8_t* cases, int all_cells);
 
 void miso_fe_destroy(miso_fe* h);
 
+/* ---------------------------------------------------------------- Flying Edges over z-slabs */
+/* The section decomposition with ghost cells of the MPI variant (marchingCubes/mpi/mpiutil.h:
+ * 87-221: each section reads one ghost plane below and two above), along z and run slab after
+ * slab on one GPU, for volumes larger than the device memory or to bound it. Slab s owns the
+ * grid planes [k0, k1), at most `planes` of them: the vertices of those planes' rows and the
+ * triangles of their cells, numbered as the whole-volume run numbers them. Concatenating the parts
+ * in order gives miso_fe_run's mesh, bit for bit. A host volume is borrowed until the last
+ * miso_fe_slabs_next and streamed to the device one slab ahead of the one being computed (from
+ * pinned memory the copy overlaps the computation); a device volume is read in place. Device
+ * memory: (planes + 3) planes of volume (twice for a host volume), about 40 bytes per grid row of
+ * those planes, and the largest part. */
+typedef struct miso_fe_slabs miso_fe_slabs;
+int miso_fe_slabs_create(const miso_volume* vol, float isoval, int64_t planes, void* stream,
+                         miso_fe_slabs** out);
+/* Extracts the next slab; *done = 1 (and nothing extracted) once all planes are done. */
+int miso_fe_slabs_next(miso_fe_slabs* h, int* done);
+/* The current slab: its planes [k0, k1), and its part of the mesh: vertices vbase .. vbase +
+ * nverts - 1 and triangles tbase .. tbase + ntris - 1 of the whole mesh. */
+int miso_fe_slabs_part(const miso_fe_slabs* h, int64_t* k0, int64_t* k1, uint64_t* vbase,
+                       uint64_t* nverts, uint64_t* tbase, uint64_t* ntris);
+/* The current part into host buffers (nverts x 3 floats twice, ntris x 3 ids that number the
+ * whole mesh's vertices), or its device buffers (valid until the next call on the handle). */
+int miso_fe_slabs_copy_part(miso_fe_slabs* h, float* points, float* normals, uint64_t* triangles);
+int miso_fe_slabs_device_part(miso_fe_slabs* h, const float** points, const float** normals,
+                              const uint64_t** triangles);
+void miso_fe_slabs_destroy(miso_fe_slabs* h);
+
 /* ---------------------------------------------------------------- Marching Cubes */
 typedef struct miso_mc miso_mc;
 

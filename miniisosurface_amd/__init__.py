@@ -22,7 +22,8 @@ from ._lib import MisoError, Volume, check, lib
 from .vtkio import Image3D, TriangleMesh, load_image, load_mesh, save_image, save_mesh
 
 __all__ = ["FlyingEdgesAlgorithm", "MarchingCubes", "DeviceVolume", "Image3D", "TriangleMesh",
-           "MisoError", "load_image", "save_image", "load_mesh", "save_mesh"]
+           "MisoError", "load_image", "save_image", "load_mesh", "save_mesh", "flying_edges",
+           "FlyingEdgesSlabs", "flying_edges_slabs"]
 
 
 @dataclass
@@ -182,6 +183,72 @@ def flying_edges(image, isoval, stream=None) -> TriangleMesh:
     algo = FlyingEdgesAlgorithm(image, isoval, stream)
     algo.run()
     return algo.moveOutput()
+
+
+class FlyingEdgesSlabs:
+    """Flying Edges over z-slabs of at most `planes` grid planes (miso_fe_slabs_*): the section
+    decomposition with ghost planes of marchingCubes/mpi/mpiutil.h:87-221 along z, one slab after
+    another on one GPU. Iterating yields, per slab, (part, points, normals, triangles) with part =
+    {k0, k1, vbase, nverts, tbase, ntris}; the triangles index the whole mesh's vertices, and the
+    parts in order concatenate to flying_edges()'s mesh. A host volume is read slab by slab (it
+    stays borrowed until the iteration ends); a DeviceVolume is read in place."""
+
+    def __init__(self, image, isoval, planes, stream=None):
+        self._h = None
+        self._vol, self._keep = _volume_struct(image)
+        h = ctypes.c_void_p()
+        check(lib().miso_fe_slabs_create(ctypes.byref(self._vol), ctypes.c_float(float(np.float32(isoval))),
+                                         int(planes), _stream_ptr(stream, bool(self._vol.on_device)),
+                                         ctypes.byref(h)), "FlyingEdgesSlabs")
+        self._h = h
+        self._destroy = lib().miso_fe_slabs_destroy
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._destroy(h)
+            self._h = None
+
+    def next(self) -> bool:
+        """Extract the next slab; False once every plane is done."""
+        done = ctypes.c_int()
+        check(lib().miso_fe_slabs_next(self._h, ctypes.byref(done)), "FlyingEdgesSlabs")
+        return not done.value
+
+    def part(self) -> dict:
+        k0, k1 = ctypes.c_int64(), ctypes.c_int64()
+        vb, nv, tb, nt = (ctypes.c_uint64() for _ in range(4))
+        check(lib().miso_fe_slabs_part(self._h, ctypes.byref(k0), ctypes.byref(k1), ctypes.byref(vb), ctypes.byref(nv),
+                                       ctypes.byref(tb), ctypes.byref(nt)), "part")
+        return dict(k0=k0.value, k1=k1.value, vbase=vb.value, nverts=nv.value, tbase=tb.value, ntris=nt.value)
+
+    def copy_part(self):
+        p = self.part()
+        pts = np.empty((p["nverts"], 3), np.float32)
+        nrm = np.empty((p["nverts"], 3), np.float32)
+        tris = np.empty((p["ntris"], 3), np.uint64)
+        check(lib().miso_fe_slabs_copy_part(self._h, pts.ctypes.data, nrm.ctypes.data, tris.ctypes.data), "copy_part")
+        return pts, nrm, tris
+
+    def device_part(self):
+        """Device pointers (points, normals, 64-bit triangles) of the current part."""
+        p, n, t = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().miso_fe_slabs_device_part(self._h, ctypes.byref(p), ctypes.byref(n), ctypes.byref(t)),
+              "device_part")
+        return p.value, n.value, t.value
+
+    def __iter__(self):
+        while self.next():
+            yield (self.part(), *self.copy_part())
+
+
+def flying_edges_slabs(image, isoval, planes, stream=None) -> TriangleMesh:
+    """flying_edges() computed slab by slab (FlyingEdgesSlabs); the same mesh."""
+    parts = list(FlyingEdgesSlabs(image, isoval, planes, stream))
+    if not parts:
+        return TriangleMesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32), np.zeros((0, 3), np.uint64))
+    return TriangleMesh(np.concatenate([q[1] for q in parts]), np.concatenate([q[2] for q in parts]),
+                        np.concatenate([q[3] for q in parts]))
 
 
 MC_ORDERS = {"first_visit": 0, "edge": 1}

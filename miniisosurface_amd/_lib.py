@@ -65,6 +65,12 @@ def lib():
         "miso_fe_pass_times": [P, P],
         "miso_fe_rows": [P, P, P, P, P, P, P, P, P],
         "miso_fe_cube_cases": [P, P, I],
+        "miso_fe_slabs_create": [ctypes.POINTER(Volume), F, ctypes.c_int64, P, PP],
+        "miso_fe_slabs_next": [P, ctypes.POINTER(I)],
+        "miso_fe_slabs_part": [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), U64P, U64P, U64P,
+                               U64P],
+        "miso_fe_slabs_copy_part": [P, P, P, P],
+        "miso_fe_slabs_device_part": [P, PP, PP, PP],
         "miso_mc_create": [ctypes.POINTER(Volume), F, P, PP],
         "miso_mc_run": [P],
         "miso_mc_counts": [P, U64P, U64P],
@@ -95,7 +101,7 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = I
-    for name in ("miso_fe_destroy", "miso_mc_destroy", "miso_batch_destroy"):
+    for name in ("miso_fe_destroy", "miso_fe_slabs_destroy", "miso_mc_destroy", "miso_batch_destroy"):
         getattr(L, name).argtypes = [P]
         getattr(L, name).restype = None
     L.miso_last_error.restype = ctypes.c_char_p

@@ -90,6 +90,10 @@ Geom make_geom(const miso_volume* v)
         g.org[a] = v->origin[a];
         g.sp[a] = v->spacing[a];
     }
+    g.kofs = 0;
+    g.nzg = v->nz;
+    g.emit_lo = 0;
+    g.emit_hi = v->ny * v->nz;
     return g;
 }
 
@@ -173,7 +177,7 @@ struct RowState {
     bool unit_geometry() const
     {
         return g.sp[0] == 1.0f && g.sp[1] == 1.0f && g.sp[2] == 1.0f && g.org[0] == 0.0f && g.org[1] == 0.0f &&
-               g.org[2] == 0.0f && g.nx < (1 << 24) && g.ny < (1 << 24) && g.nz < (1 << 24);
+               g.org[2] == 0.0f && g.nx < (1 << 24) && g.ny < (1 << 24) && g.nzg < (1 << 24);
     }
 
     /* (row, word) lane kernels: rows of up to 4096 voxels; the vertex ids a wavefront's
@@ -718,6 +722,247 @@ extern "C" void miso_fe_destroy(miso_fe* h)
     for (int e = 0; e < 8; ++e)
         if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
     delete h;
+}
+
+/* ====================================================================== Flying Edges over z-slabs */
+
+/* The volume in slabs of `planes` grid planes (the section decomposition with ghost planes of
+ * marchingCubes/mpi/mpiutil.h:87-221, here along z only and run one slab after another on one
+ * GPU). A slab owns the grid rows of planes [k0, k1): their vertices and the triangles of their
+ * cell rows. It reads planes [k0 - 1, k1 + 2): one ghost plane below for the gradient, and two
+ * above, for the z-edges and cells of plane k1 - 1 and the gradient at plane k1. The rows of
+ * plane k1 are counted again (the next slab owns them) because the slab's last cell row names
+ * their vertices. Ids continue from slab to slab, so the parts concatenate to the whole-volume
+ * mesh, bit for bit. */
+struct miso_fe_slabs {
+    Geom full{};
+    int64_t planes = 0;
+    float iso = 0.0f;
+    const float* src = nullptr;       // the whole volume, host or device
+    bool src_on_device = false;
+    RowState st;                      // sized for planes + 3 data planes
+    MeshBuffers out;                  // the current slab's part (64-bit ids)
+    float* d_stage[2] = {nullptr, nullptr};
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t copied[2] = {};
+    int64_t next_k0 = 0;
+    int64_t slab = -1;
+    int64_t k0 = 0, k1 = 0;
+    uint64_t vbase = 0, tbase = 0, vnext = 0, tnext = 0;
+    uint64_t* h_vown = nullptr;       // pinned: the slab's owned vertex count
+    bool finished = false;
+};
+
+namespace {
+
+/* Rows below lo belong to the previous slab: zeroed, so the slab's offsets start at 0. Rows from
+ * hi on keep their vertex counts (the slab's last cell row names those vertices) but their cells
+ * belong to the next slab: their triangle counts are zeroed. */
+__global__ __launch_bounds__(256) void k_slab_trim_counts(RowCounts* __restrict__ counts, int64_t R, int64_t lo,
+                                                          int64_t hi)
+{
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < R; r += (int64_t)gridDim.x * 256) {
+        if (r < lo) counts[r] = RowCounts{0u, 0u, 0u, 0u};
+        else if (r >= hi) counts[r].t = 0u;
+    }
+}
+
+int64_t slab_data_lo(int64_t k0) { return k0 > 0 ? k0 - 1 : 0; }
+int64_t slab_data_hi(const miso_fe_slabs* h, int64_t k1) { return std::min<int64_t>(k1 + 2, h->full.nz); }
+
+/* start the copy of the slab owning planes from k0 into staging buffer b (host volumes) */
+int slab_prefetch(miso_fe_slabs* h, int64_t k0, int b)
+{
+    if (h->src_on_device || k0 >= h->full.nz) return MISO_OK;
+    const int64_t k1 = std::min<int64_t>(k0 + h->planes, h->full.nz);
+    const int64_t d0 = slab_data_lo(k0), d1 = slab_data_hi(h, k1);
+    const int64_t plane = h->full.nx * h->full.ny;
+    HIPCALL(hipMemcpyAsync(h->d_stage[b], h->src + d0 * plane, (size_t)((d1 - d0) * plane) * sizeof(float),
+                           hipMemcpyHostToDevice, h->copy_stream));
+    HIPCALL(hipEventRecord(h->copied[b], h->copy_stream));
+    return MISO_OK;
+}
+
+int slab_step(miso_fe_slabs* h)
+{
+    RowState& s = h->st;
+    const Geom& G = h->full;
+    const int64_t k0 = h->next_k0, k1 = std::min<int64_t>(k0 + h->planes, G.nz);
+    const int64_t d0 = slab_data_lo(k0), d1 = slab_data_hi(h, k1);
+    const int b = (int)((h->slab + 1) & 1);
+    const int64_t plane = G.nx * G.ny;
+    if (h->src_on_device) {
+        s.d_vol = h->src + d0 * plane;
+    } else {
+        HIPCALL(hipStreamWaitEvent(s.stream, h->copied[b], 0));
+        s.d_vol = h->d_stage[b];
+        // the other buffer held the previous slab, whose kernels have finished: fill it with the next
+        if (int rc = slab_prefetch(h, k1, b ^ 1)) return rc;
+    }
+    Geom g = G;
+    g.nz = d1 - d0;
+    g.kofs = d0;
+    g.nzg = G.nz;
+    g.emit_lo = (k0 - d0) * G.ny;
+    g.emit_hi = (k1 - d0) * G.ny;
+    s.g = g;
+    const int64_t R = g.ny * g.nz;
+    s.ntiles = (R + kScanTile - 1) / kScanTile;
+    if (int rc = s.launch_classify(h->iso)) return rc;
+    if (int rc = s.launch_count()) return rc;
+    hipLaunchKernelGGL(k_slab_trim_counts, dim3((unsigned)std::min<int64_t>((R + 255) / 256, 4096)), dim3(256), 0,
+                       s.stream, s.d_counts, R, g.emit_lo, g.emit_hi);
+    HIPCALL(hipGetLastError());
+    if (int rc = s.launch_scan()) return rc;
+    if (g.emit_hi < R)
+        HIPCALL(hipMemcpyAsync(h->h_vown, s.d_voff + g.emit_hi, sizeof(uint64_t), hipMemcpyDeviceToHost, s.stream));
+    HIPCALL(hipStreamSynchronize(s.stream));
+    const uint64_t nv = g.emit_hi < R ? h->h_vown[0] : s.h_totals[0];
+    const uint64_t nt = s.h_totals[1];
+    MeshBuffers& o = h->out;
+    if (nv > o.cap_v || nt > o.cap_t || !o.d_pts || !o.d_tris) {
+        // grow with slack, so that slabs of similar size do not reallocate each time
+        if (int rc = o.reserve(nv + nv / 4 + 64, nt + nt / 4 + 64)) return rc;
+    }
+    o.nverts = nv;
+    o.ntris = nt;
+    o.index_bits = 64;
+    if (nv || nt) {
+        const uint64_t vbias = (uint64_t)0 - h->vnext;   // ids continue from the previous slabs
+        if (s.lane_rows()) {
+            if (s.unit_geometry())
+                hipLaunchKernelGGL((k_fe_emit_rows<uint64_t, true>), dim3(s.lane_row_blocks()), dim3(256), 0,
+                                   s.stream, s.d_vol, (const uint64_t*)s.d_mask, s.g, h->iso,
+                                   (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff,
+                                   (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm, (uint64_t*)o.d_tris, vbias);
+            else
+                hipLaunchKernelGGL((k_fe_emit_rows<uint64_t, false>), dim3(s.lane_row_blocks()), dim3(256), 0,
+                                   s.stream, s.d_vol, (const uint64_t*)s.d_mask, s.g, h->iso,
+                                   (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff,
+                                   (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm, (uint64_t*)o.d_tris, vbias);
+        } else {
+            hipLaunchKernelGGL(k_fe_emit<uint64_t>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                               (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,
+                               (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm,
+                               (uint64_t*)o.d_tris, vbias);
+        }
+        HIPCALL(hipGetLastError());
+    }
+    HIPCALL(hipStreamSynchronize(s.stream));
+    h->slab += 1;
+    h->k0 = k0;
+    h->k1 = k1;
+    h->vbase = h->vnext;
+    h->tbase = h->tnext;
+    h->vnext += nv;
+    h->tnext += nt;
+    h->next_k0 = k1;
+    return MISO_OK;
+}
+
+}  // namespace
+
+extern "C" void miso_fe_slabs_destroy(miso_fe_slabs* h)
+{
+    if (!h) return;
+    if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
+    h->st.release();
+    h->out.release();
+    for (int b = 0; b < 2; ++b) {
+        dev_free(h->d_stage[b]);
+        if (h->copied[b]) (void)hipEventDestroy(h->copied[b]);
+    }
+    if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
+    if (h->h_vown) (void)hipHostFree(h->h_vown);
+    delete h;
+}
+
+extern "C" int miso_fe_slabs_create(const miso_volume* vol, float isoval, int64_t planes, void* stream,
+                                    miso_fe_slabs** out)
+{
+    if (!out) return fail(MISO_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (int rc = check_volume(vol)) return rc;
+    if (planes < 1) return fail(MISO_ERR_ARG, "planes per slab must be at least 1");
+    miso_fe_slabs* h = new (std::nothrow) miso_fe_slabs();
+    if (!h) return fail(MISO_ERR_ALLOC, "host allocation failed");
+    h->full = make_geom(vol);
+    h->planes = std::min<int64_t>(planes, std::max<int64_t>(vol->nz, 1));
+    h->iso = isoval;
+    h->src = vol->data;
+    h->src_on_device = vol->on_device != 0;
+    h->finished = !has_cells(h->full);
+    // row state for the largest slab: planes + 3 data planes (never more than the volume)
+    miso_volume sv = *vol;
+    sv.nz = std::min<int64_t>(h->planes + 3, vol->nz);
+    sv.data = nullptr;
+    sv.on_device = 1;
+    int rc = h->st.init(&sv, stream);
+    const int64_t stage = sv.nz * vol->nx * vol->ny;
+    if (rc == MISO_OK && !h->src_on_device && !h->finished) {
+        for (int b = 0; b < 2 && rc == MISO_OK; ++b) {
+            rc = dev_alloc(&h->d_stage[b], (size_t)stage);
+            if (rc == MISO_OK && hipEventCreateWithFlags(&h->copied[b], hipEventDisableTiming) != hipSuccess)
+                rc = fail(MISO_ERR_HIP, "hipEventCreate failed");
+        }
+        if (rc == MISO_OK && hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess)
+            rc = fail(MISO_ERR_HIP, "hipStreamCreate failed");
+        if (rc == MISO_OK) rc = slab_prefetch(h, 0, 0);
+    }
+    if (rc == MISO_OK && hipHostMalloc((void**)&h->h_vown, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+        rc = fail(MISO_ERR_ALLOC, "hipHostMalloc failed");
+    if (rc != MISO_OK) {
+        miso_fe_slabs_destroy(h);
+        return rc;
+    }
+    h->out.force64 = true;
+    *out = h;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_slabs_next(miso_fe_slabs* h, int* done)
+{
+    if (!h || !done) return fail(MISO_ERR_ARG, "NULL argument");
+    *done = 0;
+    if (h->finished || h->next_k0 >= h->full.nz) {
+        h->finished = true;
+        *done = 1;
+        return MISO_OK;
+    }
+    if (int rc = slab_step(h)) return rc;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_slabs_part(const miso_fe_slabs* h, int64_t* k0, int64_t* k1, uint64_t* vbase,
+                                  uint64_t* nverts, uint64_t* tbase, uint64_t* ntris)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (h->slab < 0) return fail(MISO_ERR_STATE, "call miso_fe_slabs_next first");
+    if (k0) *k0 = h->k0;
+    if (k1) *k1 = h->k1;
+    if (vbase) *vbase = h->vbase;
+    if (nverts) *nverts = h->out.nverts;
+    if (tbase) *tbase = h->tbase;
+    if (ntris) *ntris = h->out.ntris;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_slabs_copy_part(miso_fe_slabs* h, float* points, float* normals, uint64_t* triangles)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (h->slab < 0) return fail(MISO_ERR_STATE, "call miso_fe_slabs_next first");
+    return h->out.copy_out(h->st.stream, points, normals, triangles);
+}
+
+extern "C" int miso_fe_slabs_device_part(miso_fe_slabs* h, const float** points, const float** normals,
+                                         const uint64_t** triangles)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (h->slab < 0) return fail(MISO_ERR_STATE, "call miso_fe_slabs_next first");
+    if (points) *points = h->out.d_pts;
+    if (normals) *normals = h->out.d_nrm;
+    if (triangles) *triangles = (const uint64_t*)h->out.d_tris;
+    return MISO_OK;
 }
 
 /* ====================================================================== Marching Cubes */

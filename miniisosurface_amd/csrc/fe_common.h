@@ -32,6 +32,11 @@ struct Geom {
     int64_t W;            // mask words per grid row
     float org[3];
     float sp[3];
+    // A z-slab of a larger volume: local plane 0 is plane kofs of a volume of nzg planes (the
+    // corner coordinates use the global plane index), and pass 4 writes only the vertices and
+    // triangles of rows [emit_lo, emit_hi). A whole volume has kofs 0, nzg nz and all rows.
+    int64_t kofs, nzg;
+    int64_t emit_lo, emit_hi;
 };
 
 struct RowCounts {
@@ -192,7 +197,7 @@ __device__ __forceinline__ void fe_edge_corners(const Geom& g, int axis, int64_t
 {
     a[0] = axis_coord(g.org[0], g.sp[0], i, g.nx);
     a[1] = axis_coord(g.org[1], g.sp[1], j, g.ny);
-    a[2] = axis_coord(g.org[2], g.sp[2], k, g.nz);
+    a[2] = axis_coord(g.org[2], g.sp[2], k + g.kofs, g.nzg);
     b[0] = a[0]; b[1] = a[1]; b[2] = a[2];
     b[axis] = a[axis] + g.sp[axis];
 }

@@ -496,13 +496,13 @@ __global__ __launch_bounds__(256) void k_fe_emit(const float* __restrict__ vol,
                                                  const uint64_t* __restrict__ voff,
                                                  const uint64_t* __restrict__ toff,
                                                  float* __restrict__ pts, float* __restrict__ nrm,
-                                                 Index* __restrict__ tris)
+                                                 Index* __restrict__ tris, uint64_t vbias = 0)
 {
     __shared__ EmitLds lds_all[kRowsPerBlock];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const int64_t r = (int64_t)blockIdx.x * kRowsPerBlock + wave;
-    if (r >= g.ny * g.nz) return;
+    if (r >= g.ny * g.nz || r < g.emit_lo || r >= g.emit_hi) return;
     const RowCounts c0 = counts[r];
     if (c0.x + c0.y + c0.z == 0 && c0.t == 0) return;
     const int64_t j = r % g.ny, k = r / g.ny;
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(256) void k_fe_emit(const float* __restrict__ vol,
                         const uint64_t a = rw.fe_vertex_id(cbase, case_edge(packed, 3 * t + 0), li);
                         const uint64_t b = rw.fe_vertex_id(cbase, case_edge(packed, 3 * t + 1), li);
                         const uint64_t c = rw.fe_vertex_id(cbase, case_edge(packed, 3 * t + 2), li);
-                        store_tri<Index>(tris, slot0 + t, a, b, c);
+                        store_tri<Index>(tris, slot0 + t, a - vbias, b - vbias, c - vbias);
                     }
                 }
                 tri_run += __shfl(incl, 63);

@@ -274,7 +274,7 @@ __device__ __forceinline__ void fe_edge_point(const float* __restrict__ v, const
     const float w = (iso - v0) / (v1 - v0);
     float A[3], B[3];
     if (Unit) {
-        A[0] = (float)i; A[1] = (float)j; A[2] = (float)k;
+        A[0] = (float)i; A[1] = (float)j; A[2] = (float)(k + (int)g.kofs);
         B[0] = A[0]; B[1] = A[1]; B[2] = A[2];
         B[axis] = A[axis] + 1.0f;
     } else {
@@ -318,7 +318,7 @@ __device__ __forceinline__ void edge_finish(const Geom& g, int axis, int i, int 
 {
     float A[3], B[3];
     if (Unit) {
-        A[0] = (float)i; A[1] = (float)j; A[2] = (float)k;
+        A[0] = (float)i; A[1] = (float)j; A[2] = (float)(k + (int)g.kofs);
         B[0] = A[0]; B[1] = A[1]; B[2] = A[2];
         B[axis] = A[axis] + 1.0f;
     } else {
@@ -443,7 +443,7 @@ struct EmitRows {
     RowLanes rl;
     int lane, wv;
     int64_t r0, r;
-    bool ok, has_y, has_z;
+    bool ok, emit, has_y, has_z;   // emit: the row is in the range pass 4 writes
     LaneWords L;
     uint64_t v0, t0, vb, eb;
     int64_t k0;   // the wavefront's first row is (j0, k0)
@@ -462,6 +462,7 @@ __device__ __forceinline__ bool emit_rows_setup(EmitRows& E, const uint64_t* __r
     if (E.r0 >= R) return false;
     E.r = E.r0 + E.rl.q;
     E.ok = E.r < R && E.rl.w < g.W;
+    E.emit = E.ok && E.r >= g.emit_lo && E.r < g.emit_hi;
     const int ny = (int)g.ny;
     const int j = (int)(E.r % ny), k = (int)(E.r / ny);
     E.has_y = j < ny - 1;
@@ -491,8 +492,10 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
     const int j0 = E.j0;
     const int64_t k0 = E.k0;
     const float* __restrict__ plane = vol + k0 * (g.nx * g.ny);
-    const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb, has_y ? ((L.m[0] ^ L.m[1]) & vb) : 0ull,
-                             has_z ? ((L.m[0] ^ L.m[2]) & vb) : 0ull};
+    const uint64_t live = E.emit ? ~0ull : 0ull;
+    const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb & live,
+                             has_y ? ((L.m[0] ^ L.m[1]) & vb & live) : 0ull,
+                             has_z ? ((L.m[0] ^ L.m[2]) & vb & live) : 0ull};
     const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
     // x and y counts share one segmented scan (row totals below 2^16)
     const uint32_t p01 = c0 | c1 << 16;
@@ -575,7 +578,7 @@ __device__ __forceinline__ void emit_rows_triangles(const EmitRows& E, EmitRowsL
     const int64_t r = E.r;
     const bool ok = E.ok, has_y = E.has_y, has_z = E.has_z;
     const uint64_t v0 = E.v0, t0 = E.t0, vb = E.vb, eb = E.eb;
-    const bool cells = ok && has_y && has_z;
+    const bool cells = ok && E.emit && has_y && has_z;
     const CornerWords C = corner_words(L);
     const uint64_t act = cells ? active_cells(C, eb) : 0ull;
     const uint32_t ncell = (uint32_t)__popcll(act);
@@ -705,13 +708,13 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
                                                       const uint64_t* __restrict__ voff,
                                                       const uint64_t* __restrict__ toff,
                                                       float* __restrict__ pts, float* __restrict__ nrm,
-                                                      Index* __restrict__ tris)
+                                                      Index* __restrict__ tris, uint64_t vbias = 0)
 {
     __shared__ uint64_t s_case[256];
     __shared__ EmitRowsLds s_w[4];
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
     __syncthreads();
-    fe_emit_rows_block<Index, Unit>(s_case, s_w, vol, mask, g, iso, counts, voff, toff, pts, nrm, tris, 0,
+    fe_emit_rows_block<Index, Unit>(s_case, s_w, vol, mask, g, iso, counts, voff, toff, pts, nrm, tris, vbias,
                                     blockIdx.x);
 }
 

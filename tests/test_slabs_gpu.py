@@ -1,0 +1,140 @@
+"""GPU: Flying Edges over z-slabs (miso_fe_slabs_*) against the whole-volume extraction.
+
+The parts, concatenated in slab order, must be the whole-volume mesh bit for bit: the reference
+fixtures and the oracle on every shape class (both kernel paths, spacing and origin, NaN, thin
+volumes), for slab heights from one plane to more than the volume, from host and from device
+volumes. Part metadata: slabs tile [0, nz), vertex and triangle ranges follow each other."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import golden_files, load_fixture
+from test_fe_gpu import CASES, assert_mesh_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def heights(nz):
+    return sorted({1, 2, 3, 7, max(nz - 1, 1), max(nz, 1), nz + 4})
+
+
+def check_parts(parts, nz, V, T):
+    """Slabs tile the planes; vertex and triangle ranges are contiguous; ids stay below V."""
+    k = 0
+    vb = tb = 0
+    for part, pts, nrm, tris in parts:
+        assert part["k0"] == k and part["k1"] > k
+        assert part["vbase"] == vb and part["tbase"] == tb
+        assert pts.shape == (part["nverts"], 3) and tris.shape == (part["ntris"], 3)
+        if tris.size:
+            assert int(tris.max()) < V
+        k, vb, tb = part["k1"], vb + part["nverts"], tb + part["ntris"]
+    assert (k, vb, tb) == ((nz if parts else k), V, T)
+
+
+def slab_mesh(image, iso, planes):
+    import miniisosurface_amd as m
+    parts = list(m.FlyingEdgesSlabs(image, iso, planes))
+    if not parts:
+        z = np.zeros((0, 3), np.float32)
+        return parts, m.TriangleMesh(z, z.copy(), np.zeros((0, 3), np.uint64))
+    return parts, m.TriangleMesh(np.concatenate([q[1] for q in parts]), np.concatenate([q[2] for q in parts]),
+                                 np.concatenate([q[3] for q in parts]))
+
+
+@pytest.mark.parametrize("path", golden_files(), ids=lambda p: os.path.basename(p)[:-4])
+def test_slabs_match_reference_fixture(path):
+    import miniisosurface_amd as m
+    vol, iso, sp, org, d = load_fixture(path)
+    nz = vol.shape[0]
+    for planes in heights(nz):
+        print(f"[slabs fixture] {os.path.basename(path)} planes={planes}", flush=True)
+        parts, mesh = slab_mesh(m.Image3D(vol, sp, org), iso, planes)
+        assert_mesh_equal(mesh, d["fe_points"], d["fe_normals"], d["fe_triangles"], f"{path} planes={planes}")
+        if nz >= 2 and vol.shape[1] >= 2 and vol.shape[2] >= 2:
+            check_parts(parts, nz, len(d["fe_points"]), len(d["fe_triangles"]))
+
+
+@pytest.mark.parametrize("name,vol,iso,sp,org", CASES, ids=[c[0] for c in CASES])
+def test_slabs_match_oracle(oracle_mod, name, vol, iso, sp, org):
+    import miniisosurface_amd as m
+    ref = oracle_mod.FEResult(vol, iso, sp, org, nthreads=4, want_debug=False)
+    for planes in heights(vol.shape[0]):
+        print(f"[slabs oracle] {name} planes={planes}", flush=True)
+        parts, mesh = slab_mesh(m.Image3D(vol, sp, org), iso, planes)
+        assert_mesh_equal(mesh, ref.points, ref.normals, ref.triangles, f"{name} planes={planes}")
+
+
+@pytest.mark.parametrize("name,vol,iso,sp,org", [c for c in CASES if c[0] in
+                                                  ("rand_67x33x21", "rand_130x7x5_sp", "rand_4097x3x2",
+                                                   "gauss_96x80x72", "noise_128")],
+                         ids=lambda c: c if isinstance(c, str) else None)
+def test_slabs_from_device_volume(oracle_mod, name, vol, iso, sp, org):
+    import torch
+
+    import miniisosurface_amd as m
+    ref = oracle_mod.FEResult(vol, iso, sp, org, nthreads=4, want_debug=False)
+    t = torch.from_numpy(np.ascontiguousarray(vol)).cuda()
+    dv = m.DeviceVolume.from_tensor(t, sp, org)
+    for planes in (1, 5, vol.shape[0]):
+        print(f"[slabs device] {name} planes={planes}", flush=True)
+        parts, mesh = slab_mesh(dv, iso, planes)
+        assert_mesh_equal(mesh, ref.points, ref.normals, ref.triangles, f"{name} planes={planes}")
+        check_parts(parts, vol.shape[0], ref.nverts, ref.ntris)
+
+
+def test_slabs_device_parts_equal_whole_run_256():
+    """256^3 value noise, 37-plane slabs: each part's device buffers equal the matching slices of
+    the whole-volume run's device output (compared on the GPU)."""
+    import torch
+
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+    vol = synth.value_noise(256, seed=5, device="cuda")
+    dv = m.DeviceVolume.from_tensor(vol)
+    whole = m.FlyingEdgesAlgorithm(dv, 0.5)
+    whole.set_index_bits(64)
+    whole.run()
+    wp, wn, wt, bits, V, T = whole.device_output()
+    assert bits == 64
+
+    def dev(ptr, n, dtype, cols=3):
+        # a torch view of library-owned device memory, for the comparison only
+        return torch.as_tensor(_DeviceArray(ptr, n * cols, dtype), device="cuda").view(n, cols)
+
+    slabs = m.FlyingEdgesSlabs(dv, 0.5, 37)
+    seen = 0
+    while slabs.next():
+        part = slabs.part()
+        pp, pn, pt = slabs.device_part()
+        vb, nv, tb, nt = part["vbase"], part["nverts"], part["tbase"], part["ntris"]
+        a = dev(pp, nv, torch.float32)
+        assert torch.equal(a.view(torch.int32), dev(wp, V, torch.float32)[vb:vb + nv].view(torch.int32))
+        assert torch.equal(dev(pn, nv, torch.float32).view(torch.int32),
+                           dev(wn, V, torch.float32)[vb:vb + nv].view(torch.int32))
+        assert torch.equal(dev(pt, nt, torch.int64), dev(wt, T, torch.int64)[tb:tb + nt])
+        seen += 1
+    assert seen == 7 and (slabs.part()["vbase"] + slabs.part()["nverts"]) == V
+
+
+class _DeviceArray:
+    """__cuda_array_interface__ over a raw device pointer (no ownership)."""
+
+    def __init__(self, ptr, n, dtype):
+        import torch
+        typestr = {torch.float32: "<f4", torch.int64: "<i8"}[dtype]
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False), "version": 3}
+
+
+def test_slabs_errors_and_empty_volumes():
+    import miniisosurface_amd as m
+    vol = np.random.default_rng(1).uniform(0, 1, (6, 5, 4)).astype(np.float32)
+    with pytest.raises(m.MisoError):
+        m.FlyingEdgesSlabs(m.Image3D(vol), 0.5, 0)
+    s = m.FlyingEdgesSlabs(m.Image3D(vol), 0.5, 2)
+    with pytest.raises(m.MisoError):
+        s.part()                                   # before the first slab
+    for shape in ((1, 5, 4), (6, 1, 4), (6, 5, 1), (0, 0, 0)):
+        parts = list(m.FlyingEdgesSlabs(m.Image3D(np.zeros(shape, np.float32)), 0.5, 2))
+        assert parts == [] or all(p[0]["nverts"] == 0 and p[0]["ntris"] == 0 for p in parts)
