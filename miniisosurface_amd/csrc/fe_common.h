@@ -71,6 +71,15 @@ __device__ __forceinline__ T load_vox(const float* __restrict__ p)
     return t;
 }
 
+/* 4 voxels in one 16-byte load marked non-temporal: data streamed once (the whole volume in
+ * pass 1) is not kept in the caches. 4-byte alignment suffices. */
+typedef float Vox4Stream __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ Vox4 load_vox4_stream(const float* __restrict__ p)
+{
+    const Vox4Stream q = __builtin_nontemporal_load(reinterpret_cast<const Vox4Stream*>(p));
+    return Vox4{q.x, q.y, q.z, q.w};
+}
+
 /* bits [0, count) of a 64-bit word, count clipped to [0, 64] */
 __device__ __forceinline__ uint64_t low_bits(int64_t count)
 {

@@ -30,7 +30,7 @@ constexpr uint64_t kValMask = (1ull << 62) - 1;
 
 /* ------------------------------------------------------------------ pass 1a: sign mask */
 /* Rows 4 blk .. 4 blk + 3, one per wavefront. Each lane loads 4 consecutive voxels (one 16-byte
- * load, so a wavefront step covers 256 voxels = 4 mask words); its 4 sign bits are placed at
+ * non-temporal load, so a wavefront step covers 256 voxels = 4 mask words); its 4 sign bits are placed at
  * bit 4 * (lane % 16) and OR-reduced over each 16-lane row with DPP, so lane 16 t + 15 ends up
  * holding mask word t of the step. Voxels past nx read as below the isovalue. Rows of at most
  * 256 voxels take one voxel per lane and __ballot instead (fewer instructions per short row). */
@@ -68,7 +68,7 @@ __device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol,
             const int64_t i = c0 + 256 * u + 4 * lane;
             float v[4];
             if (i + 4 <= g.nx) {
-                const Vox4 q = load_vox<Vox4>(row + i);
+                const Vox4 q = load_vox4_stream(row + i);
                 v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
             } else {
 #pragma unroll
