@@ -215,9 +215,10 @@ template <typename Index>
 __device__ __forceinline__ void store_tri(Index* __restrict__ tris, uint64_t slot, uint64_t a,
                                           uint64_t b, uint64_t c)
 {
-    tris[3 * slot + 0] = (Index)a;
-    tris[3 * slot + 1] = (Index)b;
-    tris[3 * slot + 2] = (Index)c;
+    // non-temporal: the triangle stream is written once and not read back here
+    __builtin_nontemporal_store((Index)a, tris + 3 * slot + 0);
+    __builtin_nontemporal_store((Index)b, tris + 3 * slot + 1);
+    __builtin_nontemporal_store((Index)c, tris + 3 * slot + 2);
 }
 
 }  // namespace miso

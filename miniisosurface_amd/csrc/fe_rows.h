@@ -451,6 +451,7 @@ struct EmitRows {
 };
 
 __device__ __forceinline__ bool emit_rows_setup(EmitRows& E, const uint64_t* __restrict__ mask, const Geom& g,
+                                                const RowCounts* __restrict__ counts,
                                                 const uint64_t* __restrict__ voff, const uint64_t* __restrict__ toff,
                                                 int64_t blk)
 {
@@ -462,6 +463,16 @@ __device__ __forceinline__ bool emit_rows_setup(EmitRows& E, const uint64_t* __r
     if (E.r0 >= R) return false;
     E.r = E.r0 + E.rl.q;
     E.ok = E.r < R && E.rl.w < g.W;
+    {
+        // a wavefront whose rows have no vertex and no triangle leaves before reading mask words
+        // (most of them at a sparse isovalue)
+        bool some = false;
+        if (E.ok && E.rl.w == 0) {
+            const RowCounts c = counts[E.r];
+            some = (c.x | c.y | c.z | c.t) != 0u;
+        }
+        if (__ballot(some) == 0ull) return false;
+    }
     E.emit = E.ok && E.r >= g.emit_lo && E.r < g.emit_hi;
     const int ny = (int)g.ny;
     const int j = (int)(E.r % ny), k = (int)(E.r / ny);
@@ -696,7 +707,7 @@ __device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ 
                                                    uint64_t vbias, int64_t blk, const Index* __restrict__ remap = nullptr)
 {
     EmitRows E;
-    if (!emit_rows_setup(E, mask, g, voff, toff, blk)) return;
+    if (!emit_rows_setup(E, mask, g, counts, voff, toff, blk)) return;
     if (!Remap) emit_rows_vertices<Unit>(E, s_w[E.wv].u.vlist, vol, g, iso, pts, nrm);
     emit_rows_triangles<Index, Remap>(E, s_w[E.wv], s_case, g, counts, voff, tris, vbias, remap);
 }
