@@ -159,8 +159,8 @@ def bench_slabs(vol, iso, planes=128, reps=3):
     nz, ny, nx = vol.shape
     cells = (nx - 1) * (ny - 1) * (nz - 1)
 
-    def run(image, copy):
-        s = m.FlyingEdgesSlabs(image, iso, planes)
+    def run(s, copy):
+        s.restart(iso)
         n = 0
         while s.next():
             if copy:
@@ -170,11 +170,13 @@ def bench_slabs(vol, iso, planes=128, reps=3):
         return n, p["vbase"] + p["nverts"], p["tbase"] + p["ntris"]
 
     def timed(image, copy):
-        run(image, copy)
+        # one handle (device buffers allocated once), restarted for every extraction
+        s = m.FlyingEdgesSlabs(image, iso, planes)
+        run(s, copy)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            res = run(image, copy)
+            res = run(s, copy)
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps, res
 

@@ -115,6 +115,9 @@ int miso_fe_slabs_create(const miso_volume* vol, float isoval, int64_t planes, v
                          miso_fe_slabs** out);
 /* Extracts the next slab; *done = 1 (and nothing extracted) once all planes are done. */
 int miso_fe_slabs_next(miso_fe_slabs* h, int* done);
+/* Starts over from the first slab at a new isovalue, keeping the device buffers (the same
+ * volume, still borrowed). */
+int miso_fe_slabs_restart(miso_fe_slabs* h, float isoval);
 /* The current slab: its planes [k0, k1), and its part of the mesh: vertices vbase .. vbase +
  * nverts - 1 and triangles tbase .. tbase + ntris - 1 of the whole mesh. */
 int miso_fe_slabs_part(const miso_fe_slabs* h, int64_t* k0, int64_t* k1, uint64_t* vbase,

@@ -215,6 +215,10 @@ class FlyingEdgesSlabs:
         check(lib().miso_fe_slabs_next(self._h, ctypes.byref(done)), "FlyingEdgesSlabs")
         return not done.value
 
+    def restart(self, isoval):
+        """Start over from the first slab at a new isovalue (the device buffers are kept)."""
+        check(lib().miso_fe_slabs_restart(self._h, ctypes.c_float(float(np.float32(isoval)))), "restart")
+
     def part(self) -> dict:
         k0, k1 = ctypes.c_int64(), ctypes.c_int64()
         vb, nv, tb, nt = (ctypes.c_uint64() for _ in range(4))

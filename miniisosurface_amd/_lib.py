@@ -67,6 +67,7 @@ def lib():
         "miso_fe_cube_cases": [P, P, I],
         "miso_fe_slabs_create": [ctypes.POINTER(Volume), F, ctypes.c_int64, P, PP],
         "miso_fe_slabs_next": [P, ctypes.POINTER(I)],
+        "miso_fe_slabs_restart": [P, F],
         "miso_fe_slabs_part": [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), U64P, U64P, U64P,
                                U64P],
         "miso_fe_slabs_copy_part": [P, P, P, P],

@@ -733,23 +733,30 @@ extern "C" void miso_fe_destroy(miso_fe* h)
  * above, for the z-edges and cells of plane k1 - 1 and the gradient at plane k1. The rows of
  * plane k1 are counted again (the next slab owns them) because the slab's last cell row names
  * their vertices. Ids continue from slab to slab, so the parts concatenate to the whole-volume
- * mesh, bit for bit. */
+ * mesh, bit for bit.
+ *
+ * Two slots, each with its own row state and stream, alternate: while slab s writes its part,
+ * slab s + 1 is classified, counted and scanned on the other slot's stream (and, for a host
+ * volume, copied into that slot's staging buffer first). */
 struct miso_fe_slabs {
     Geom full{};
     int64_t planes = 0;
     float iso = 0.0f;
     const float* src = nullptr;       // the whole volume, host or device
     bool src_on_device = false;
-    RowState st;                      // sized for planes + 3 data planes
-    MeshBuffers out;                  // the current slab's part (64-bit ids)
+    RowState st[2];                   // per slot, each sized for planes + 3 data planes
     float* d_stage[2] = {nullptr, nullptr};
+    hipEvent_t copied[2] = {};        // a slot's staging buffer holds its slab's planes
+    hipEvent_t emitted[2] = {};       // a slot's part is written (its buffers may be reused)
+    bool emit_pending[2] = {false, false};
+    uint64_t* h_vown = nullptr;       // pinned, one per slot: the slab's owned vertex count
     hipStream_t copy_stream = nullptr;
-    hipEvent_t copied[2] = {};
-    int64_t next_k0 = 0;
-    int64_t slab = -1;
+    MeshBuffers out;                  // the current slab's part (64-bit ids)
+    int64_t slot_k0[2] = {0, 0}, slot_k1[2] = {0, 0};
+    int64_t counted_k1 = 0;           // planes whose counts have been launched end here
+    int64_t slab = -1;                // the current slab (its part is in `out`)
     int64_t k0 = 0, k1 = 0;
     uint64_t vbase = 0, tbase = 0, vnext = 0, tnext = 0;
-    uint64_t* h_vown = nullptr;       // pinned: the slab's owned vertex count
     bool finished = false;
 };
 
@@ -770,34 +777,23 @@ __global__ __launch_bounds__(256) void k_slab_trim_counts(RowCounts* __restrict_
 int64_t slab_data_lo(int64_t k0) { return k0 > 0 ? k0 - 1 : 0; }
 int64_t slab_data_hi(const miso_fe_slabs* h, int64_t k1) { return std::min<int64_t>(k1 + 2, h->full.nz); }
 
-/* start the copy of the slab owning planes from k0 into staging buffer b (host volumes) */
-int slab_prefetch(miso_fe_slabs* h, int64_t k0, int b)
+/* Passes 1-3 of the slab owning planes [k0, min(k0 + planes, nz)) on slot p, with the D2H copies
+ * of its totals; the slot's previous work (the part it last wrote) must have finished. */
+int slab_launch_counts(miso_fe_slabs* h, int p, int64_t k0)
 {
-    if (h->src_on_device || k0 >= h->full.nz) return MISO_OK;
-    const int64_t k1 = std::min<int64_t>(k0 + h->planes, h->full.nz);
-    const int64_t d0 = slab_data_lo(k0), d1 = slab_data_hi(h, k1);
-    const int64_t plane = h->full.nx * h->full.ny;
-    HIPCALL(hipMemcpyAsync(h->d_stage[b], h->src + d0 * plane, (size_t)((d1 - d0) * plane) * sizeof(float),
-                           hipMemcpyHostToDevice, h->copy_stream));
-    HIPCALL(hipEventRecord(h->copied[b], h->copy_stream));
-    return MISO_OK;
-}
-
-int slab_step(miso_fe_slabs* h)
-{
-    RowState& s = h->st;
+    RowState& s = h->st[p];
     const Geom& G = h->full;
-    const int64_t k0 = h->next_k0, k1 = std::min<int64_t>(k0 + h->planes, G.nz);
+    const int64_t k1 = std::min<int64_t>(k0 + h->planes, G.nz);
     const int64_t d0 = slab_data_lo(k0), d1 = slab_data_hi(h, k1);
-    const int b = (int)((h->slab + 1) & 1);
     const int64_t plane = G.nx * G.ny;
     if (h->src_on_device) {
         s.d_vol = h->src + d0 * plane;
     } else {
-        HIPCALL(hipStreamWaitEvent(s.stream, h->copied[b], 0));
-        s.d_vol = h->d_stage[b];
-        // the other buffer held the previous slab, whose kernels have finished: fill it with the next
-        if (int rc = slab_prefetch(h, k1, b ^ 1)) return rc;
+        HIPCALL(hipMemcpyAsync(h->d_stage[p], h->src + d0 * plane, (size_t)((d1 - d0) * plane) * sizeof(float),
+                               hipMemcpyHostToDevice, h->copy_stream));
+        HIPCALL(hipEventRecord(h->copied[p], h->copy_stream));
+        HIPCALL(hipStreamWaitEvent(s.stream, h->copied[p], 0));
+        s.d_vol = h->d_stage[p];
     }
     Geom g = G;
     g.nz = d1 - d0;
@@ -815,10 +811,28 @@ int slab_step(miso_fe_slabs* h)
     HIPCALL(hipGetLastError());
     if (int rc = s.launch_scan()) return rc;
     if (g.emit_hi < R)
-        HIPCALL(hipMemcpyAsync(h->h_vown, s.d_voff + g.emit_hi, sizeof(uint64_t), hipMemcpyDeviceToHost, s.stream));
+        HIPCALL(hipMemcpyAsync(h->h_vown + p, s.d_voff + g.emit_hi, sizeof(uint64_t), hipMemcpyDeviceToHost, s.stream));
+    h->slot_k0[p] = k0;
+    h->slot_k1[p] = k1;
+    h->counted_k1 = k1;
+    return MISO_OK;
+}
+
+/* Pass 4 of the slab counted on slot p, into `out`, once its counts are back. */
+int slab_emit(miso_fe_slabs* h, int p)
+{
+    RowState& s = h->st[p];
     HIPCALL(hipStreamSynchronize(s.stream));
-    const uint64_t nv = g.emit_hi < R ? h->h_vown[0] : s.h_totals[0];
+    const Geom& g = s.g;
+    const int64_t R = g.ny * g.nz;
+    const uint64_t nv = g.emit_hi < R ? h->h_vown[p] : s.h_totals[0];
     const uint64_t nt = s.h_totals[1];
+    // the previous part (written on the other slot) is no longer needed; its kernels must be done
+    // before its buffers are reused or reallocated
+    if (h->emit_pending[p ^ 1]) {
+        HIPCALL(hipEventSynchronize(h->emitted[p ^ 1]));
+        h->emit_pending[p ^ 1] = false;
+    }
     MeshBuffers& o = h->out;
     if (nv > o.cap_v || nt > o.cap_t || !o.d_pts || !o.d_tris) {
         // grow with slack, so that slabs of similar size do not reallocate each time
@@ -848,15 +862,26 @@ int slab_step(miso_fe_slabs* h)
         }
         HIPCALL(hipGetLastError());
     }
-    HIPCALL(hipStreamSynchronize(s.stream));
+    HIPCALL(hipEventRecord(h->emitted[p], s.stream));
+    h->emit_pending[p] = true;
     h->slab += 1;
-    h->k0 = k0;
-    h->k1 = k1;
+    h->k0 = h->slot_k0[p];
+    h->k1 = h->slot_k1[p];
     h->vbase = h->vnext;
     h->tbase = h->tnext;
     h->vnext += nv;
     h->tnext += nt;
-    h->next_k0 = k1;
+    return MISO_OK;
+}
+
+/* waits for the current part */
+int slab_part_ready(miso_fe_slabs* h)
+{
+    const int p = (int)(h->slab & 1);
+    if (h->emit_pending[p]) {
+        HIPCALL(hipEventSynchronize(h->emitted[p]));
+        h->emit_pending[p] = false;
+    }
     return MISO_OK;
 }
 
@@ -866,11 +891,12 @@ extern "C" void miso_fe_slabs_destroy(miso_fe_slabs* h)
 {
     if (!h) return;
     if (h->copy_stream) (void)hipStreamSynchronize(h->copy_stream);
-    h->st.release();
+    for (int p = 0; p < 2; ++p) h->st[p].release();
     h->out.release();
-    for (int b = 0; b < 2; ++b) {
-        dev_free(h->d_stage[b]);
-        if (h->copied[b]) (void)hipEventDestroy(h->copied[b]);
+    for (int p = 0; p < 2; ++p) {
+        dev_free(h->d_stage[p]);
+        if (h->copied[p]) (void)hipEventDestroy(h->copied[p]);
+        if (h->emitted[p]) (void)hipEventDestroy(h->emitted[p]);
     }
     if (h->copy_stream) (void)hipStreamDestroy(h->copy_stream);
     if (h->h_vown) (void)hipHostFree(h->h_vown);
@@ -897,20 +923,26 @@ extern "C" int miso_fe_slabs_create(const miso_volume* vol, float isoval, int64_
     sv.nz = std::min<int64_t>(h->planes + 3, vol->nz);
     sv.data = nullptr;
     sv.on_device = 1;
-    int rc = h->st.init(&sv, stream);
+    int rc = MISO_OK;
+    for (int p = 0; p < 2 && rc == MISO_OK; ++p) {
+        rc = h->st[p].init(&sv, stream);
+        if (rc == MISO_OK && hipEventCreateWithFlags(&h->emitted[p], hipEventDisableTiming) != hipSuccess)
+            rc = fail(MISO_ERR_HIP, "hipEventCreate failed");
+    }
     const int64_t stage = sv.nz * vol->nx * vol->ny;
     if (rc == MISO_OK && !h->src_on_device && !h->finished) {
-        for (int b = 0; b < 2 && rc == MISO_OK; ++b) {
-            rc = dev_alloc(&h->d_stage[b], (size_t)stage);
-            if (rc == MISO_OK && hipEventCreateWithFlags(&h->copied[b], hipEventDisableTiming) != hipSuccess)
+        for (int p = 0; p < 2 && rc == MISO_OK; ++p) {
+            rc = dev_alloc(&h->d_stage[p], (size_t)stage);
+            if (rc == MISO_OK && hipEventCreateWithFlags(&h->copied[p], hipEventDisableTiming) != hipSuccess)
                 rc = fail(MISO_ERR_HIP, "hipEventCreate failed");
         }
         if (rc == MISO_OK && hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking) != hipSuccess)
             rc = fail(MISO_ERR_HIP, "hipStreamCreate failed");
-        if (rc == MISO_OK) rc = slab_prefetch(h, 0, 0);
     }
-    if (rc == MISO_OK && hipHostMalloc((void**)&h->h_vown, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+    if (rc == MISO_OK && hipHostMalloc((void**)&h->h_vown, 2 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
         rc = fail(MISO_ERR_ALLOC, "hipHostMalloc failed");
+    // the first slab's passes 1-3 start at once
+    if (rc == MISO_OK && !h->finished) rc = slab_launch_counts(h, 0, 0);
     if (rc != MISO_OK) {
         miso_fe_slabs_destroy(h);
         return rc;
@@ -924,12 +956,33 @@ extern "C" int miso_fe_slabs_next(miso_fe_slabs* h, int* done)
 {
     if (!h || !done) return fail(MISO_ERR_ARG, "NULL argument");
     *done = 0;
-    if (h->finished || h->next_k0 >= h->full.nz) {
+    if (h->finished || (h->slab >= 0 && h->k1 >= h->full.nz)) {
         h->finished = true;
         *done = 1;
         return MISO_OK;
     }
-    if (int rc = slab_step(h)) return rc;
+    const int p = (int)((h->slab + 1) & 1);      // the slot whose counts are in flight
+    if (int rc = slab_emit(h, p)) return rc;
+    // the next slab's passes 1-3 on the other slot, behind its last part (already waited for)
+    if (h->counted_k1 < h->full.nz)
+        if (int rc = slab_launch_counts(h, p ^ 1, h->counted_k1)) return rc;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_slabs_restart(miso_fe_slabs* h, float isoval)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    for (int p = 0; p < 2; ++p) HIPCALL(hipStreamSynchronize(h->st[p].stream));
+    if (h->copy_stream) HIPCALL(hipStreamSynchronize(h->copy_stream));
+    h->emit_pending[0] = h->emit_pending[1] = false;
+    h->iso = isoval;
+    h->slab = -1;
+    h->k0 = h->k1 = 0;
+    h->vbase = h->tbase = h->vnext = h->tnext = 0;
+    h->counted_k1 = 0;
+    h->finished = !has_cells(h->full);
+    if (!h->finished)
+        if (int rc = slab_launch_counts(h, 0, 0)) return rc;
     return MISO_OK;
 }
 
@@ -951,7 +1004,8 @@ extern "C" int miso_fe_slabs_copy_part(miso_fe_slabs* h, float* points, float* n
 {
     if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
     if (h->slab < 0) return fail(MISO_ERR_STATE, "call miso_fe_slabs_next first");
-    return h->out.copy_out(h->st.stream, points, normals, triangles);
+    if (int rc = slab_part_ready(h)) return rc;
+    return h->out.copy_out(h->st[h->slab & 1].stream, points, normals, triangles);
 }
 
 extern "C" int miso_fe_slabs_device_part(miso_fe_slabs* h, const float** points, const float** normals,
@@ -959,6 +1013,7 @@ extern "C" int miso_fe_slabs_device_part(miso_fe_slabs* h, const float** points,
 {
     if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
     if (h->slab < 0) return fail(MISO_ERR_STATE, "call miso_fe_slabs_next first");
+    if (int rc = slab_part_ready(h)) return rc;
     if (points) *points = h->out.d_pts;
     if (normals) *normals = h->out.d_nrm;
     if (triangles) *triangles = (const uint64_t*)h->out.d_tris;

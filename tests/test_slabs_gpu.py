@@ -138,3 +138,23 @@ def test_slabs_errors_and_empty_volumes():
     for shape in ((1, 5, 4), (6, 1, 4), (6, 5, 1), (0, 0, 0)):
         parts = list(m.FlyingEdgesSlabs(m.Image3D(np.zeros(shape, np.float32)), 0.5, 2))
         assert parts == [] or all(p[0]["nverts"] == 0 and p[0]["ntris"] == 0 for p in parts)
+
+
+def test_slabs_restart_at_other_isovalues(oracle_mod):
+    """One handle restarted at several isovalues (buffers kept): each pass equals the oracle."""
+    import miniisosurface_amd as m
+    vol = np.random.default_rng(8).uniform(0, 1, (23, 17, 29)).astype(np.float32)
+    s = m.FlyingEdgesSlabs(m.Image3D(vol), 0.5, 4)
+    for iso in (0.5, 0.3, 0.8, 0.5):
+        s.restart(iso)
+        parts = list(s)
+        ref = oracle_mod.FEResult(vol, iso, want_debug=False)
+        mesh = m.TriangleMesh(np.concatenate([q[1] for q in parts]), np.concatenate([q[2] for q in parts]),
+                              np.concatenate([q[3] for q in parts]))
+        assert_mesh_equal(mesh, ref.points, ref.normals, ref.triangles, f"iso {iso}")
+    s.restart(0.5)
+    assert s.next()                     # restart in the middle of an iteration
+    s.restart(0.3)
+    parts = list(s)
+    ref = oracle_mod.FEResult(vol, 0.3, want_debug=False)
+    assert sum(q[0]["nverts"] for q in parts) == ref.nverts
