@@ -318,6 +318,15 @@ struct McState {
                                (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff, (const uint64_t*)d_xyz,
                                (const uint64_t*)(d_axstate + 3 * sw), R, (Index*)d_map);
             const uint64_t* no_moff = nullptr;
+            if (s.lane_rows() && s.unit_geometry()) {
+                // Flying Edges pass 4 through the permutation
+                hipLaunchKernelGGL(k_mc_emit_rows<Index>, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,
+                                   (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
+                                   (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, (const Index*)d_map, pts,
+                                   nrm, tris);
+                HIPCALL(hipGetLastError());
+                return MISO_OK;
+            }
             if (!s.lane_rows()) {
                 hipLaunchKernelGGL((k_mc_vertices<Index, true>), dim3(s.row_blocks()), dim3(256), 0, s.stream,
                                    s.d_vol, (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,
@@ -347,6 +356,8 @@ struct McState {
             return MISO_OK;
         }
         if (s.lane_rows()) {
+            // (the map alone followed by k_mc_emit_rows, as edge order runs, measures the same here:
+            // first-visit ids scatter the Flying Edges emit's vertex writes)
             if (s.unit_geometry())
                 hipLaunchKernelGGL((k_mc_vertices_rows<Index, true>), dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
                                    s.d_vol, (const uint64_t*)s.d_mask, s.g, iso, (const RowCounts*)s.d_counts,

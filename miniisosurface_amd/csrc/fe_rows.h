@@ -489,11 +489,11 @@ __device__ __forceinline__ bool emit_rows_setup(EmitRows& E, const uint64_t* __r
 }
 
 /* Vertices of the wavefront's rows, [x][y][z] cuts per row, through an LDS list of kEmitVertCap
- * entries. Voxel offsets are 32-bit from plane k0. */
-template <bool Unit>
+ * entries. Voxel offsets are 32-bit from plane k0. With vmap, vertex id v is written at vmap[v]. */
+template <bool Unit, typename Index = uint32_t>
 __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* list, const float* __restrict__ vol,
                                                    const Geom& g, float iso, float* __restrict__ pts,
-                                                   float* __restrict__ nrm)
+                                                   float* __restrict__ nrm, const Index* __restrict__ vmap = nullptr)
 {
     const RowLanes& rl = E.rl;
     const LaneWords& L = E.L;
@@ -563,7 +563,8 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
                 const uint32_t ent = list[e];
                 float pp[3], nn[3];
                 emit_vertex<Unit>(plane, g, iso, ent, j0, (int)k0, wp_log2, grp == 0, pp, nn);
-                const uint64_t vid = v0 + (ent >> 14);
+                uint64_t vid = v0 + (ent >> 14);
+                if (vmap) vid = (uint64_t)vmap[vid];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     pts[3 * vid + c] = pp[c];
@@ -727,6 +728,28 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
     __syncthreads();
     fe_emit_rows_block<Index, Unit>(s_case, s_w, vol, mask, g, iso, counts, voff, toff, pts, nrm, tris, vbias,
                                     blockIdx.x);
+}
+
+/* Marching Cubes for unit spacing and zero origin, where a Marching Cubes vertex has the bits of
+ * the Flying Edges vertex on the same edge: the Flying Edges pass 4 with every vertex written at
+ * fe_to_mc[its id] and every triangle's ids read through the same map. */
+template <typename Index>
+__global__ __launch_bounds__(256) void k_mc_emit_rows(const float* __restrict__ vol,
+                                                      const uint64_t* __restrict__ mask, Geom g, float iso,
+                                                      const RowCounts* __restrict__ counts,
+                                                      const uint64_t* __restrict__ voff,
+                                                      const uint64_t* __restrict__ toff,
+                                                      const Index* __restrict__ fe_to_mc, float* __restrict__ pts,
+                                                      float* __restrict__ nrm, Index* __restrict__ tris)
+{
+    __shared__ uint64_t s_case[256];
+    __shared__ EmitRowsLds s_w[4];
+    s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
+    __syncthreads();
+    EmitRows E;
+    if (!emit_rows_setup(E, mask, g, counts, voff, toff, blockIdx.x)) return;
+    emit_rows_vertices<true, Index>(E, s_w[E.wv].u.vlist, vol, g, iso, pts, nrm, fe_to_mc);
+    emit_rows_triangles<Index, true>(E, s_w[E.wv], s_case, g, counts, voff, tris, 0, fe_to_mc);
 }
 
 /* Marching Cubes triangles: the Flying Edges triangle stream through the id map. */
