@@ -331,6 +331,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-slabs", action="store_true")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="only the 512^3 gyroid section (Flying Edges and Marching Cubes), for profiling")
     ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--calibrate", action="store_true",
                     help="run one plain 1 GiB device copy first (calibrates rocprofv3 byte counters)")
@@ -352,6 +354,9 @@ def main():
     import miniisosurface_amd as m
     from miniisosurface_amd import synth
 
+    if args.c2_only:
+        print(json.dumps({"c2_gyroid_512_iso0": bench_c2(reps=max(args.steps, 1))}))
+        return
     n = args.n
     torch.cuda.init()
     stream = torch.cuda.current_stream()

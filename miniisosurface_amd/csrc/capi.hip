@@ -314,7 +314,7 @@ struct McState {
                 hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)ntiles), dim3(256), 0, s.stream,
                                    (const uint32_t*)s.d_counts + a, R, d_xyz + a * R, d_axstate + a * sw, ntiles,
                                    d_axstate + 3 * sw + a, (int64_t)4);
-            hipLaunchKernelGGL(k_edge_order_perm<Index>, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s.stream,
+            hipLaunchKernelGGL(k_edge_order_perm<Index>, dim3((unsigned)((R + 15) / 16)), dim3(256), 0, s.stream,
                                (const RowCounts*)s.d_counts, (const uint64_t*)s.d_voff, (const uint64_t*)d_xyz,
                                (const uint64_t*)(d_axstate + 3 * sw), R, (Index*)d_map);
             const uint64_t* no_moff = nullptr;

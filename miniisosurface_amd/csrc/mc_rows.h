@@ -131,9 +131,8 @@ __device__ __forceinline__ void mc_corners(const Geom& g, int64_t i, int64_t j, 
  * vertices sorted by global edge id, marchingCubes/util/Image3D.cpp:170-253): all cut x-edges in
  * (k, j, i) order, then all y-edges, then all z-edges. perm maps each Flying Edges vertex id
  * (row blocks [x][y][z]) to that order; xyz holds the per-row exclusive scans of the x, y and z
- * counts (R each) and tot their totals. A block takes 256 rows: their offsets and bases go to
- * LDS, then each thread takes every 256th Flying Edges id of the rows' contiguous id range and
- * finds its row by binary search (coalesced stores, every lane busy whatever the row lengths). */
+ * counts (R each) and tot their totals. Sixteen lanes per row, four rows per wavefront: the
+ * lanes run over the row's ids [x][y][z] together (64-byte store runs). */
 template <typename Index>
 __global__ __launch_bounds__(256) void k_edge_order_perm(const RowCounts* __restrict__ counts,
                                                          const uint64_t* __restrict__ voff,
@@ -141,39 +140,18 @@ __global__ __launch_bounds__(256) void k_edge_order_perm(const RowCounts* __rest
                                                          const uint64_t* __restrict__ tot, int64_t R,
                                                          Index* __restrict__ perm)
 {
-    __shared__ uint64_t s_off[257];
-    __shared__ uint64_t s_base[3][256];
-    __shared__ uint32_t s_cx[256], s_cxy[256];
-    const int t = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * 256;
-    const int nrows = (int)(R - r0 < 256 ? R - r0 : 256);
-    if (t < nrows) {
-        const int64_t r = r0 + t;
-        const RowCounts c = counts[r];
-        const uint64_t off = voff[r];
-        s_off[t] = off;
-        s_base[0][t] = xyz[r];
-        s_base[1][t] = tot[0] + xyz[R + r];
-        s_base[2][t] = tot[0] + tot[1] + xyz[2 * R + r];
-        s_cx[t] = c.x;
-        s_cxy[t] = c.x + c.y;
-        if (t == nrows - 1) s_off[nrows] = off + c.x + c.y + c.z;
-    }
-    __syncthreads();
-    const uint64_t f0 = s_off[0], f1 = s_off[nrows];
-    for (uint64_t f = f0 + (uint64_t)t; f < f1; f += 256) {
-        // the row holding id f: the last q with s_off[q] <= f (a row with no vertex is never it,
-        // since the next row starts at the same offset)
-        int lo = 0, hi = nrows - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= f) lo = mid;
-            else hi = mid - 1;
-        }
-        const uint32_t o = (uint32_t)(f - s_off[lo]);
-        const int a = o < s_cx[lo] ? 0 : (o < s_cxy[lo] ? 1 : 2);
-        const uint32_t before = a == 0 ? 0u : (a == 1 ? s_cx[lo] : s_cxy[lo]);
-        perm[f] = (Index)(s_base[a][lo] + (o - before));
+    const int64_t r = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    if (r >= R) return;
+    const uint32_t l = threadIdx.x & 15;
+    const RowCounts c = counts[r];
+    const uint32_t n = c.x + c.y + c.z;
+    if (n == 0) return;
+    const uint64_t f0 = voff[r];
+    const uint64_t bx = xyz[r], by = tot[0] + xyz[R + r], bz = tot[0] + tot[1] + xyz[2 * R + r];
+    const uint32_t cxy = c.x + c.y;
+    for (uint32_t t = l; t < n; t += 16) {
+        const uint64_t id = t < c.x ? bx + t : (t < cxy ? by + (t - c.x) : bz + (t - cxy));
+        perm[f0 + t] = (Index)id;
     }
 }
 
