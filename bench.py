@@ -197,6 +197,34 @@ def bench_slabs(vol, iso, planes=128, reps=3):
     return out
 
 
+def bench_large(reps=5):
+    """2048^3 gyroid (16 periods, no noise), iso 0.1: 8.6 G voxels (32 GiB), volume resident in HBM,
+    one complete extraction per step (the scale check beyond 2^32 voxels)."""
+    import torch
+
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+
+    n = 2048
+    vol = synth.gyroid(n, seed=0, periods=16, noise=0.0, device="cuda")
+    algo = m.FlyingEdgesAlgorithm(m.DeviceVolume.from_tensor(vol), 0.1)
+    algo.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        algo.run()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    V, T = algo.counts()
+    b_alg = 4 * n ** 3 + 24 * V + 12 * T
+    out = {"ms_per_step": round(wall * 1e3, 3), "gcells_s": round((n - 1) ** 3 / wall / 1e9, 2),
+           "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4), "nverts": V, "ntris": T,
+           "pass_ms": [round(float(x), 3) for x in algo.pass_times_ms()]}
+    del algo, vol
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_batched(reps):
     """256 mixed-size Gaussian volumes (dims log-uniform in [32, 256], 1-6 blobs, iso U[0.2, 0.8],
     seed 0), and one 512^3 gyroid at 16 isovalues in [-1, 1]: the batch (one persistent launch per
@@ -331,6 +359,7 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-slabs", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--c2-only", action="store_true",
                     help="only the 512^3 gyroid section (Flying Edges and Marching Cubes), for profiling")
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -454,6 +483,8 @@ def main():
     }
     if not args.no_slabs:
         out["slabs_128_planes"] = bench_slabs(vol, args.iso)
+    if not args.no_large:
+        out["large_2048_gyroid_iso0.1"] = bench_large()
     if not args.no_c2:
         out["c2_gyroid_512_iso0"] = bench_c2()
     if not args.no_batch:
