@@ -4,7 +4,7 @@
  * Plain C types only: pointers, sizes, status codes. No exception crosses this boundary; every
  * call returns MISO_OK (0) or a negative status, and miso_last_error() describes the last failure
  * on the calling thread. One handle is used from one host thread at a time; each handle owns one
- * HIP stream (or borrows the caller's).
+ * HIP stream (or borrows the caller's); a slab handle owns two, plus one for host-to-device copies.
  *
  * Reference interfaces these entry points replace (paths relative to the miniIsosurface root):
  *   FlyingEdgesAlgorithm(Image3D const&, scalar_t const&)   flyingEdges/serial/FlyingEdgesAlgorithm.h:29-39
@@ -15,6 +15,9 @@
  *                                                            marchingCubes/serial/main.cpp:42-44
  *   MarchingCubes(image, isoval, nSectionsX, nSectionsY, nSectionsZ)
  *                                                            marchingCubes/openmp/main.cpp:159-162
+ *   the duplicate-free variant's MarchingCubes and util::duplicateRemover
+ *                                                            marchingCubes/openmpDupFree/main.cpp:159-331
+ *   loadImageSections (sections with ghost cells)            marchingCubes/mpi/mpiutil.h:87-221
  */
 #ifndef MINIISO_H
 #define MINIISO_H
@@ -107,9 +110,10 @@ void miso_fe_destroy(miso_fe* h);
  * triangles of their cells, numbered as the whole-volume run numbers them. Concatenating the parts
  * in order gives miso_fe_run's mesh, bit for bit. A host volume is borrowed until the last
  * miso_fe_slabs_next and streamed to the device one slab ahead of the one being computed (from
- * pinned memory the copy overlaps the computation); a device volume is read in place. Device
- * memory: (planes + 3) planes of volume (twice for a host volume), about 40 bytes per grid row of
- * those planes, and the largest part. */
+ * pinned memory the copy overlaps the computation); a device volume is read in place. While the
+ * caller takes slab s's part, slab s + 1 is already being counted. Device memory: (planes + 3)
+ * planes of volume twice for a host volume, two sets of row state for (planes + 3) planes (about
+ * 40 bytes per grid row each, plus the sign mask), and the largest part. */
 typedef struct miso_fe_slabs miso_fe_slabs;
 int miso_fe_slabs_create(const miso_volume* vol, float isoval, int64_t planes, void* stream,
                          miso_fe_slabs** out);
