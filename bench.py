@@ -23,20 +23,19 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(vol_np, iso, threads):
-    """The reference's OpenMP Flying Edges (compiled from its sources into oracle/_ref) on a
-    bounded slab of the same volume; falls back to the oracle's OpenMP restatement."""
+def _reference_fe_seconds(exe_name, vol_np, iso, threads, runs):
+    """Median wall time of the reference Flying Edges driver (compiled from its sources into
+    oracle/_ref) on vol_np, by its own timer (ctor .. moveOutput, file I/O excluded); falls back
+    to the oracle's restatement of the same loops. Returns (seconds, kind)."""
     from oracle import ora
     from miniisosurface_amd import vtkio
-    nz, ny, nx = vol_np.shape
-    cells = (nx - 1) * (ny - 1) * (nz - 1)
-    exe = ora.ref_binary("flyingEdgesOpenMP")
+    exe = ora.ref_binary(exe_name)
     times = []
     if exe:
         with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
             vtkio.save_image(os.path.join(tmp, "v.vtk"), vtkio.Image3D(vol_np))
             env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
-            for _ in range(3):
+            for _ in range(runs):
                 out = subprocess.run([exe, "-i", "v.vtk", "-o", "m.vtk", "-v", repr(float(iso)),
                                       "-y", os.path.join(tmp, "y", "r.yaml")], cwd=tmp, env=env,
                                      check=True, capture_output=True, text=True).stdout
@@ -45,14 +44,55 @@ def cpu_baseline(vol_np, iso, threads):
                         times.append(float(line.split(":")[-1]))
         kind = "reference"
     else:
-        for _ in range(3):
+        for _ in range(runs):
             r = ora.FEResult(vol_np, iso, nthreads=threads, want_debug=False)
             times.append(r.times[5])
         kind = "port"
-    t = sorted(times)[len(times) // 2]
+    return sorted(times)[len(times) // 2], kind
+
+
+def cpu_baseline(vol_np, iso, threads, runs=5):
+    """The reference's OpenMP Flying Edges on a bounded slab of the same volume, all threads."""
+    nz, ny, nx = vol_np.shape
+    cells = (nx - 1) * (ny - 1) * (nz - 1)
+    t, kind = _reference_fe_seconds("flyingEdgesOpenMP", vol_np, iso, threads, runs)
     return {"value": round(cells / t / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": kind,
-            "sample": f"{nx}x{ny}x{nz} slab of the same volume, iso {iso}, median of 3 runs, "
-                      f"timer = reference ctor..moveOutput ({t:.3f} s)"}
+            "sample": f"{nx}x{ny}x{nz} slab of the same volume, iso {iso}, median of {runs} runs, "
+                      f"OMP_PROC_BIND=close, OMP_PLACES=cores, timer = reference ctor..moveOutput ({t:.3f} s)"}
+
+
+def bench_c1(threads, runs=5, reps=20):
+    """The reference's own CPU case: 128^3, 4 Gaussians (sigma 8-20, seed 0), iso 0.5. The
+    reference serial and OpenMP Flying Edges on the host beside one extraction on the GPU (host
+    volume already on the device: launch-latency bound, 8.9 MB), and the GPU call with its
+    host-to-device copy and the copy of the mesh back."""
+    import torch
+
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+    vol = synth.gaussians(128, 4, 0)
+    cells = 127 ** 3
+    ts, kind = _reference_fe_seconds("flyingEdgesSerial", vol, 0.5, 1, runs)
+    to, _ = _reference_fe_seconds("flyingEdgesOpenMP", vol, 0.5, threads, runs)
+    algo = m.FlyingEdgesAlgorithm(m.Image3D(vol), 0.5)
+    for _ in range(3):
+        algo.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        algo.run()
+    torch.cuda.synchronize()
+    gpu = (time.perf_counter() - t0) / reps
+    m.flying_edges(m.Image3D(vol), 0.5)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        mesh = m.flying_edges(m.Image3D(vol), 0.5)
+    whole = (time.perf_counter() - t0) / 5
+    return {"nverts": mesh.numberOfVertices(), "ntris": mesh.numberOfTriangles(), "kind": kind,
+            "reference_serial_ms": round(ts * 1e3, 3), "reference_openmp_ms": round(to * 1e3, 3),
+            "openmp_threads": threads, "gpu_ms": round(gpu * 1e3, 4),
+            "gpu_with_copies_ms": round(whole * 1e3, 3),
+            "gpu_gcells_s": round(cells / gpu / 1e9, 2)}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fe_hbm_traffic_1024_iso0.5.txt")
@@ -493,7 +533,10 @@ def main():
         out["serving"] = bench_serving(args.serve_requests)
     if not args.no_cpu:
         slab = vol[: max(2, n // 4)].cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(slab, args.iso, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = cpu_baseline(slab, args.iso, threads)
+        out["cpu_baseline_1_thread"] = cpu_baseline(slab[: max(2, n // 16)], args.iso, 1)
+        out["c1_gaussians_128_iso0.5"] = bench_c1(threads)
     print(json.dumps(out))
 
 
