@@ -265,6 +265,28 @@ def bench_large(reps=5):
     return out
 
 
+def bench_host_scope(vol, iso, reps=2):
+    """The reference driver's timed scope (ctor .. moveOutput, flyingEdges/serial/main.cpp:110-158)
+    from a host volume: the host-to-device copy of the volume, the four passes and the mesh copied
+    back into host arrays with 64-bit indices (the reference's size_t). PCIe-inclusive; not `value`."""
+    import torch
+
+    import miniisosurface_amd as m
+    host = vol.cpu().numpy()
+    nz, ny, nx = host.shape
+    cells = (nx - 1) * (ny - 1) * (nz - 1)
+    image = m.Image3D(host)
+    m.flying_edges(image, iso)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mesh = m.flying_edges(image, iso)
+    wall = (time.perf_counter() - t0) / reps
+    return {"ms_per_step": round(wall * 1e3, 2), "gcells_s": round(cells / wall / 1e9, 3),
+            "nverts": mesh.numberOfVertices(), "ntris": mesh.numberOfTriangles(),
+            "volume_bytes": int(host.nbytes), "mesh_bytes": int(mesh.points.nbytes * 2 + mesh.triangles.nbytes)}
+
+
 def bench_batched(reps):
     """256 mixed-size Gaussian volumes (dims log-uniform in [32, 256], 1-6 blobs, iso U[0.2, 0.8],
     seed 0), and one 512^3 gyroid at 16 isovalues in [-1, 1]: the batch (one persistent launch per
@@ -400,6 +422,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-slabs", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-host-scope", action="store_true")
     ap.add_argument("--c2-only", action="store_true",
                     help="only the 512^3 gyroid section (Flying Edges and Marching Cubes), for profiling")
     ap.add_argument("--serve-requests", type=int, default=1000)
@@ -521,6 +544,8 @@ def main():
         "sparse_iso_0.95": sparse,
         "volume_gen_s": round(gen_s, 2),
     }
+    if not args.no_host_scope:
+        out["reference_scope_host_volume"] = bench_host_scope(vol, args.iso)
     if not args.no_slabs:
         out["slabs_128_planes"] = bench_slabs(vol, args.iso)
     if not args.no_large:
