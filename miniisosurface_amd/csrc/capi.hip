@@ -504,7 +504,9 @@ struct miso_fe {
 
 namespace {
 
-int fe_pass_launch(miso_fe* h, int pass)
+/* need: pass 4 launched before its output size is known on the host (pass 3's totals still on the
+ * device), into the buffers of an earlier run; the kernel writes nothing if they do not fit. */
+int fe_pass_launch(miso_fe* h, int pass, const uint64_t* need = nullptr)
 {
     RowState& s = h->st;
     if (!has_cells(s.g)) return MISO_OK;
@@ -521,11 +523,14 @@ int fe_pass_launch(miso_fe* h, int pass)
         break;
     case 4: {
         const MeshBuffers& o = h->out;
-        if ((o.nverts || o.ntris) && s.lane_rows()) {
+        const bool work = need || o.nverts || o.ntris;
+        const uint64_t cap_t = o.index_bits ? o.cap_t * (uint64_t)o.cap_t_bits / (uint64_t)o.index_bits : 0;
+        if (work && s.lane_rows()) {
 #define MISO_EMIT_ROWS(IDX, UNIT)                                                                                  \
     hipLaunchKernelGGL((k_fe_emit_rows<IDX, UNIT>), dim3(s.lane_row_blocks()), dim3(256), 0, s.stream, s.d_vol,    \
                        (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,                     \
-                       (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm, (IDX*)o.d_tris)
+                       (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm, (IDX*)o.d_tris,   \
+                       (uint64_t)0, need, o.cap_v, cap_t)
             const bool unit = s.unit_geometry();
             if (o.index_bits == 64) {
                 if (unit) MISO_EMIT_ROWS(uint64_t, true);
@@ -536,17 +541,17 @@ int fe_pass_launch(miso_fe* h, int pass)
             }
 #undef MISO_EMIT_ROWS
             HIPCALL(hipGetLastError());
-        } else if (o.nverts || o.ntris) {
+        } else if (work) {
             if (o.index_bits == 64)
                 hipLaunchKernelGGL(k_fe_emit<uint64_t>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
                                    (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,
                                    (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm,
-                                   (uint64_t*)o.d_tris);
+                                   (uint64_t*)o.d_tris, (uint64_t)0, need, o.cap_v, cap_t);
             else
                 hipLaunchKernelGGL(k_fe_emit<uint32_t>, dim3(s.row_blocks()), dim3(256), 0, s.stream, s.d_vol,
                                    (const uint64_t*)s.d_mask, s.g, h->iso, (const RowCounts*)s.d_counts,
                                    (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, o.d_pts, o.d_nrm,
-                                   (uint32_t*)o.d_tris);
+                                   (uint32_t*)o.d_tris, (uint64_t)0, need, o.cap_v, cap_t);
             HIPCALL(hipGetLastError());
         }
         break;
@@ -630,6 +635,23 @@ extern "C" int miso_fe_run(miso_fe* h)
     if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
     for (int p = 1; p <= 3; ++p)
         if (int rc = fe_pass_launch(h, p)) return rc;
+    MeshBuffers& o = h->out;
+    if (has_cells(h->st.g) && o.d_pts && o.d_tris && o.index_bits) {
+        // Buffers from an earlier run: pass 4 follows pass 3 at once, without a host round trip,
+        // and checks on the device that the totals fit. Only if they do not (a larger mesh, or
+        // other index width) are the buffers grown and pass 4 run again.
+        if (int rc = fe_pass_launch(h, 4, h->st.d_totals)) return rc;
+        HIPCALL(hipStreamSynchronize(h->st.stream));
+        const uint64_t v = h->st.h_totals[0], t = h->st.h_totals[1];
+        const int bits = (o.force64 || v > 0xFFFFFFFFull) ? 64 : 32;
+        const uint64_t cap_t = o.cap_t * (uint64_t)o.cap_t_bits / (uint64_t)o.index_bits;
+        if (v <= o.cap_v && t <= cap_t && bits == o.index_bits) {
+            o.nverts = v;
+            o.ntris = t;
+            h->done = 4;
+            return fe_collect_times(h);
+        }
+    }
     if (int rc = fe_after_scan(h)) return rc;
     if (int rc = fe_pass_launch(h, 4)) return rc;
     HIPCALL(hipStreamSynchronize(h->st.stream));

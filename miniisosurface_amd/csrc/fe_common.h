@@ -211,6 +211,14 @@ __device__ __forceinline__ void fe_edge_corners(const Geom& g, int axis, int64_t
     b[axis] = a[axis] + g.sp[axis];
 }
 
+/* pass 3's totals {V, T} fit output buffers of cap_v vertices and cap_t triangles of Index ids */
+template <typename Index>
+__device__ __forceinline__ bool output_fits(const uint64_t* __restrict__ need, uint64_t cap_v, uint64_t cap_t)
+{
+    const uint64_t v = need[0], t = need[1];
+    return v <= cap_v && t <= cap_t && (sizeof(Index) == 8 || v <= 0xFFFFFFFFull);
+}
+
 template <typename Index>
 __device__ __forceinline__ void store_tri(Index* __restrict__ tris, uint64_t slot, uint64_t a,
                                           uint64_t b, uint64_t c)

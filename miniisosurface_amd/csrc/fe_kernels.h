@@ -496,8 +496,11 @@ __global__ __launch_bounds__(256) void k_fe_emit(const float* __restrict__ vol,
                                                  const uint64_t* __restrict__ voff,
                                                  const uint64_t* __restrict__ toff,
                                                  float* __restrict__ pts, float* __restrict__ nrm,
-                                                 Index* __restrict__ tris, uint64_t vbias = 0)
+                                                 Index* __restrict__ tris, uint64_t vbias = 0,
+                                                 const uint64_t* __restrict__ need = nullptr, uint64_t cap_v = 0,
+                                                 uint64_t cap_t = 0)
 {
+    if (need && !output_fits<Index>(need, cap_v, cap_t)) return;   // as k_fe_emit_rows
     __shared__ EmitLds lds_all[kRowsPerBlock];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;

@@ -720,8 +720,13 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
                                                       const uint64_t* __restrict__ voff,
                                                       const uint64_t* __restrict__ toff,
                                                       float* __restrict__ pts, float* __restrict__ nrm,
-                                                      Index* __restrict__ tris, uint64_t vbias = 0)
+                                                      Index* __restrict__ tris, uint64_t vbias = 0,
+                                                      const uint64_t* __restrict__ need = nullptr,
+                                                      uint64_t cap_v = 0, uint64_t cap_t = 0)
 {
+    // with need (pass 3's totals, on the device), write nothing unless they fit the buffers and
+    // the index width (the host then grows the buffers and runs this pass again)
+    if (need && !output_fits<Index>(need, cap_v, cap_t)) return;
     __shared__ uint64_t s_case[256];
     __shared__ EmitRowsLds s_w[4];
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];

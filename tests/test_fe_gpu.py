@@ -128,6 +128,28 @@ def test_fe_handle_reuse_across_isovalues(oracle_mod):
         assert_mesh_equal(mesh, ref.points, ref.normals, ref.triangles, f"iso={iso}")
 
 
+@pytest.mark.parametrize("case", ["rand_67x33x21", "rand_5000x4x3_sp"])
+def test_fe_rerun_grows_and_switches_index_width(oracle_mod, case):
+    """Repeated run() on one handle writes pass 4 into the previous buffers, checked on the
+    device; a larger mesh or another index width must fall back to growing them. Both kernel
+    paths (rows of at most 4096 voxels, and wider)."""
+    import miniisosurface_amd as m
+    name, vol, iso, sp, org = [c for c in CASES if c[0] == case][0]
+    algo = m.FlyingEdgesAlgorithm(m.Image3D(vol, sp, org), iso)
+    steps = ((iso, None), (iso, None), (0.95, None), (iso, None), (iso, 64), (iso, 64), (iso, 32), (0.05, None))
+    forced64 = False
+    for s_iso, bits in steps:
+        if bits:
+            algo.set_index_bits(bits)
+            forced64 = bits == 64
+        algo.set_isoval(s_iso)
+        algo.run()
+        ref = oracle_mod.FEResult(vol, s_iso, sp, org, want_debug=False)
+        mesh = algo.moveOutput()
+        assert_mesh_equal(mesh, ref.points, ref.normals, ref.triangles, f"{name} iso={s_iso} bits={bits}")
+        assert algo.device_output()[3] == (64 if forced64 else 32)
+
+
 def test_fe_device_resident_volume(oracle_mod):
     import torch
     import miniisosurface_amd as m
