@@ -19,10 +19,10 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import MisoError, Volume, check, lib
-from .vtkio import Image3D, TriangleMesh, load_image, load_mesh, save_image, save_mesh
+from .vtkio import Image3D, TriangleMesh, load_dat_image, load_image, load_mesh, save_image, save_mesh
 
 __all__ = ["FlyingEdgesAlgorithm", "MarchingCubes", "DeviceVolume", "Image3D", "TriangleMesh",
-           "MisoError", "load_image", "save_image", "load_mesh", "save_mesh", "flying_edges",
+           "MisoError", "load_image", "load_dat_image", "save_image", "load_mesh", "save_mesh", "flying_edges",
            "FlyingEdgesSlabs", "flying_edges_slabs"]
 
 

@@ -118,6 +118,24 @@ def save_image(path: str, img: Image3D, vtk_type: str = "float") -> None:
         f.write(img.data.astype(_VTK_TYPES[vtk_type]).tobytes())
 
 
+def load_dat_image(path: str) -> Image3D:
+    """loadDatImage (marchingCubes/util/LoadImage.h:177-206): three little-endian uint16
+    dimensions, then nx*ny*nz little-endian uint16 samples, x fastest; unit spacing, zero origin.
+    Samples missing at the end of a short file read as 0, as the reference's failed reads leave
+    them."""
+    if not os.path.exists(path):
+        raise FileNotFound(path)
+    raw = np.fromfile(path, dtype="<u2")
+    if raw.size < 3:
+        raise BadFormat(f"{path}: no dimensions")
+    nx, ny, nz = (int(d) for d in raw[:3])
+    n = nx * ny * nz
+    data = np.zeros(n, np.float32)
+    got = raw[3:3 + n]
+    data[:got.size] = got
+    return Image3D(data.reshape(nz, ny, nx))
+
+
 def load_image(path: str) -> Image3D:
     """util::loadImage (flyingEdges/util/LoadImage.h:168-198)."""
     if not os.path.exists(path):

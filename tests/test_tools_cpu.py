@@ -162,3 +162,28 @@ def test_c_abi_exports_every_declared_symbol():
     assert not missing, missing
     L.miso_version.restype = ctypes.c_char_p
     assert L.miso_version().startswith(b"miniiso")
+
+
+def test_dat_image_loader():
+    """loadDatImage's format (marchingCubes/util/LoadImage.h:177-206): uint16 LE dims, then
+    uint16 LE samples x fastest; a short file leaves the missing samples 0. The C++ tools' reader
+    is checked by tests/test_cli_gpu.py::test_marching_cubes_cli_dat_input."""
+    from miniisosurface_amd import load_dat_image
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 7, 5, 3
+    samples = rng.integers(0, 65536, nx * ny * nz).astype("<u2")
+    with tempfile.TemporaryDirectory() as tmp:
+        p = os.path.join(tmp, "v.dat")
+        with open(p, "wb") as f:
+            f.write(np.array([nx, ny, nz], "<u2").tobytes())
+            f.write(samples.tobytes())
+        img = load_dat_image(p)
+        assert img.dims == (nx, ny, nz) and img.spacing == (1.0, 1.0, 1.0) and img.origin == (0.0, 0.0, 0.0)
+        assert np.array_equal(img.data[1, 2, 3], np.float32(samples[3 + nx * (2 + ny * 1)]))
+        assert np.array_equal(img.data.ravel(), samples.astype(np.float32))
+        with open(p, "wb") as f:          # short file: the last 10 samples missing
+            f.write(np.array([nx, ny, nz], "<u2").tobytes())
+            f.write(samples[:-10].tobytes())
+        img = load_dat_image(p)
+        assert np.array_equal(img.data.ravel()[:-10], samples[:-10].astype(np.float32))
+        assert not img.data.ravel()[-10:].any()
