@@ -713,6 +713,31 @@ __device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ 
     emit_rows_triangles<Index, Remap>(E, s_w[E.wv], s_case, g, counts, voff, tris, vbias, remap);
 }
 
+/* The order in which pass 4's blocks take the rows, across the 8 XCDs. The hardware hands blocks
+ * to the XCDs round-robin (block b to XCD b mod 8); this gives each XCD tiles of TG consecutive
+ * row blocks (about 64 grid rows) by TP = 3 planes instead of every eighth block, so that the
+ * rows around a wavefront's own that its vertices read (j - 1 .. j + 2, k - 1 .. k + 2) are
+ * mostly in the same XCD's L2, while all XCDs still move through the planes together (they share
+ * the MALL). At 1024^3 pass 4 takes 1.65 ms in this order against 1.85 ms in dispatch order.
+ * Identity when the blocks do not split the planes evenly. A bijection on [0, n) otherwise:
+ * block b is the i-th (i = b / 8) of XCD x = b % 8, which takes tile t = i / (TG TP) of its
+ * column of tiles, TP planes by TG blocks, planes first. */
+__device__ __forceinline__ int64_t emit_block_order(int64_t b, int64_t n, const Geom& g)
+{
+    const int64_t per_block = 4 * row_lanes(g.W).rpw;
+    if (g.ny % per_block) return b;
+    const int64_t G = g.ny / per_block;                     // blocks per plane
+    int64_t TG = per_block >= 64 ? 1 : 64 / per_block;
+    while (TG > 1 && G % (8 * TG)) TG >>= 1;
+    if (G % (8 * TG)) return b;
+    constexpr int64_t TP = 3;
+    const int64_t n0 = n / (G * TP) * (G * TP);             // whole tile rows; the rest in order
+    if (b >= n0) return b;
+    const int64_t x = b % 8, i = b / 8, U = G / (8 * TG), per_tile = TG * TP;
+    const int64_t t = i % per_tile, u = (i / per_tile) % U, c = i / (per_tile * U);
+    return (c * TP + t % TP) * G + (x + 8 * u) * TG + t / TP;
+}
+
 template <typename Index, bool Unit>
 __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ vol,
                                                       const uint64_t* __restrict__ mask, Geom g, float iso,
@@ -732,7 +757,7 @@ __global__ __launch_bounds__(256) void k_fe_emit_rows(const float* __restrict__ 
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
     __syncthreads();
     fe_emit_rows_block<Index, Unit>(s_case, s_w, vol, mask, g, iso, counts, voff, toff, pts, nrm, tris, vbias,
-                                    blockIdx.x);
+                                    emit_block_order(blockIdx.x, gridDim.x, g));
 }
 
 /* Marching Cubes for unit spacing and zero origin, where a Marching Cubes vertex has the bits of
