@@ -777,7 +777,7 @@ __global__ __launch_bounds__(256) void k_mc_emit_rows(const float* __restrict__ 
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
     __syncthreads();
     EmitRows E;
-    if (!emit_rows_setup(E, mask, g, counts, voff, toff, blockIdx.x)) return;
+    if (!emit_rows_setup(E, mask, g, counts, voff, toff, emit_block_order(blockIdx.x, gridDim.x, g))) return;
     emit_rows_vertices<true, Index>(E, s_w[E.wv].u.vlist, vol, g, iso, pts, nrm, fe_to_mc);
     emit_rows_triangles<Index, true>(E, s_w[E.wv], s_case, g, counts, voff, tris, 0, fe_to_mc);
 }
@@ -796,7 +796,7 @@ __global__ __launch_bounds__(256) void k_mc_triangles_rows(const uint64_t* __res
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
     __syncthreads();
     fe_emit_rows_block<Index, true, true>(s_case, s_w, nullptr, mask, g, 0.0f, counts, voff, toff, nullptr, nullptr,
-                                          tris, 0, blockIdx.x, fe_to_mc);
+                                          tris, 0, emit_block_order(blockIdx.x, gridDim.x, g), fe_to_mc);
 }
 
 }  // namespace miso

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
     McRowsLds& S = s_w[wv];
     const RowLanes rl = row_lanes(g.W);
     const int64_t R = g.ny * g.nz;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wv) * rl.rpw;
+    const int64_t r0 = (emit_block_order(blockIdx.x, gridDim.x, g) * 4 + wv) * rl.rpw;   // Flying Edges pass 4's order
     if (r0 >= R) return;
     const int64_t r = r0 + rl.q;
     const bool ok = r < R && rl.w < g.W;
