@@ -67,6 +67,12 @@ def _cases():
     yield "rand_4096x3x3", rng.uniform(0, 1, (3, 3, 4096)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "rand_4097x3x2", rng.uniform(0, 1, (2, 3, 4097)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "rand_5000x4x3_sp", rng.uniform(0, 1, (3, 4, 5000)).astype(np.float32), 0.4, (0.5, 2.0, 1.5), (1.0, -2.0, 0.25)
+    # rows of 9 to 16 mask words (4 rows per wavefront, 16 lanes per row, some lanes past the row
+    # end), and blocks that run past the last row
+    yield "rand_600x5x4", rng.uniform(0, 1, (4, 5, 600)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
+    yield "rand_1024x7x6_sp", rng.uniform(0, 1, (6, 7, 1024)).astype(np.float32), 0.45, (0.8, 1.25, 0.6), (2.0, -1.0, 0.5)
+    yield "gauss_777x40x12", synth.gaussians((777, 40, 12), 5, 21, sigma_range=(4.0, 12.0)), 0.4, (1, 1, 1), (0, 0, 0)
+    yield "rand_513x33x3", rng.uniform(0, 1, (3, 33, 513)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "checker_64", (np.indices((64, 64, 64)).sum(0) % 2).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "gauss_96x80x72", synth.gaussians((96, 80, 72), 6, 3, sigma_range=(5.0, 14.0)), 0.3, (1, 1, 1), (0, 0, 0)
     yield "gyroid_128", synth.gyroid(128, 1), 0.0, (1, 1, 1), (0, 0, 0)
