@@ -94,6 +94,7 @@ Geom make_geom(const miso_volume* v)
     g.nzg = v->nz;
     g.emit_lo = 0;
     g.emit_hi = v->ny * v->nz;
+    g.bo = plan_block_order(g.ny, g.nz, g.W);
     return g;
 }
 
@@ -834,6 +835,7 @@ int slab_launch_counts(miso_fe_slabs* h, int p, int64_t k0)
     g.nzg = G.nz;
     g.emit_lo = (k0 - d0) * G.ny;
     g.emit_hi = (k1 - d0) * G.ny;
+    g.bo = plan_block_order(g.ny, g.nz, g.W);
     s.g = g;
     const int64_t R = g.ny * g.nz;
     s.ntiles = (R + kScanTile - 1) / kScanTile;

@@ -27,6 +27,13 @@
 
 namespace miso {
 
+/* The order in which the lane-row kernels' blocks take the grid rows (plan_block_order in
+ * fe_rows.h): blocks [0, n0) go in tiles of 2^lg_tg blocks by 3 planes of G blocks, U tiles per
+ * XCD across a plane; the rest, and every block when n0 is 0, in dispatch order. */
+struct BlockOrder {
+    uint32_t n0, G, U, lg_tg;
+};
+
 struct Geom {
     int64_t nx, ny, nz;
     int64_t W;            // mask words per grid row
@@ -37,6 +44,7 @@ struct Geom {
     // triangles of rows [emit_lo, emit_hi). A whole volume has kofs 0, nzg nz and all rows.
     int64_t kofs, nzg;
     int64_t emit_lo, emit_hi;
+    BlockOrder bo;        // planned for ny, nz and W whenever they are set
 };
 
 struct RowCounts {

@@ -73,6 +73,10 @@ def _cases():
     yield "rand_1024x7x6_sp", rng.uniform(0, 1, (6, 7, 1024)).astype(np.float32), 0.45, (0.8, 1.25, 0.6), (2.0, -1.0, 0.5)
     yield "gauss_777x40x12", synth.gaussians((777, 40, 12), 5, 21, sigma_range=(4.0, 12.0)), 0.4, (1, 1, 1), (0, 0, 0)
     yield "rand_513x33x3", rng.uniform(0, 1, (3, 33, 513)).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
+    # pass 4 row blocks in the XCD tile order: 256-row blocks, 8 per plane, 48 of 56 blocks in
+    # whole tile rows; 16-row blocks, tiles of 2 blocks, every block in a tile row
+    yield "gauss_64x2048x7", synth.gaussians((64, 2048, 7), 4, 31, sigma_range=(3.0, 9.0)), 0.35, (1, 1, 1), (0, 0, 0)
+    yield "rand_1024x256x9_sp", rng.uniform(0, 1, (9, 256, 1024)).astype(np.float32), 0.5, (0.5, 1.5, 1.0), (0.0, 1.0, -2.0)
     yield "checker_64", (np.indices((64, 64, 64)).sum(0) % 2).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "gauss_96x80x72", synth.gaussians((96, 80, 72), 6, 3, sigma_range=(5.0, 14.0)), 0.3, (1, 1, 1), (0, 0, 0)
     yield "gyroid_128", synth.gyroid(128, 1), 0.0, (1, 1, 1), (0, 0, 0)
