@@ -30,14 +30,41 @@ struct RowLanes {
 __device__ __forceinline__ RowLanes row_lanes(int64_t W)
 {
     RowLanes L;
-    int wp = 1;
-    while (wp < W) wp <<= 1;
-    L.Wp = wp;
-    L.rpw = 64 / wp;
+    const int lg = W <= 1 ? 0 : 64 - __builtin_clzll((unsigned long long)(W - 1));   // Wp = 2^lg >= W
+    L.Wp = 1 << lg;
+    L.rpw = 64 >> lg;
     const int lane = lane_id();
-    L.q = lane / wp;
-    L.w = lane % wp;
+    L.q = lane >> lg;
+    L.w = lane & (L.Wp - 1);
     return L;
+}
+
+/* This wavefront's index within its 256-thread block, in a scalar register. */
+__device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+/* a / b for values that are the same in every lane (scalar code; 32-bit when both fit) */
+__device__ __forceinline__ int64_t uniform_div(int64_t a, int64_t b)
+{
+    if (((uint64_t)a | (uint64_t)b) >> 32 == 0) return (int64_t)((uint32_t)a / (uint32_t)b);
+    return a / b;
+}
+
+/* (j, k) of grid row r0 + q, for a wavefront whose first row r0 is the same in every lane and
+ * 0 <= q < 64: one division per wavefront, then carries per lane (at most one once ny >= 64). */
+struct RowJK {
+    int64_t j, k;
+};
+
+__device__ __forceinline__ RowJK split_row(int64_t r0, int q, int64_t ny)
+{
+    RowJK p;
+    p.k = uniform_div(r0, ny);
+    p.j = r0 - p.k * ny + q;
+    while (p.j >= ny) {
+        p.j -= ny;
+        ++p.k;
+    }
+    return p;
 }
 
 /* Prefix scans and reductions over the Wp lanes of one row segment (Wp a power of two, the
@@ -82,7 +109,7 @@ template <typename Op>
 __device__ __forceinline__ uint32_t seg_reduce(uint32_t v, int Wp)
 {
     const int lane = lane_id();
-    const uint32_t inc = seg_incl<Op>(v, Wp, lane % Wp);
+    const uint32_t inc = seg_incl<Op>(v, Wp, lane & (Wp - 1));
     return (uint32_t)__shfl(inc, (lane & ~(Wp - 1)) + Wp - 1);
 }
 
@@ -192,11 +219,12 @@ __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__
                                                     int64_t blk)
 {
     const RowLanes rl = row_lanes(g.W);
-    const int64_t wave_id = blk * 4 + (threadIdx.x >> 6);
-    const int64_t r = wave_id * rl.rpw + rl.q;
+    const int64_t r0 = (blk * 4 + wave_in_block()) * rl.rpw;
+    const int64_t r = r0 + rl.q;
     const int64_t R = g.ny * g.nz;
     const bool ok = r < R && rl.w < g.W;
-    const int64_t j = r % g.ny, k = r / g.ny;
+    const RowJK jk = split_row(r0, rl.q, g.ny);
+    const int64_t j = jk.j, k = jk.k;
     const bool has_y = j < g.ny - 1, has_z = k < g.nz - 1;
     const LaneWords L = load_lane_words(mask, g, r, rl.w, ok, has_y, has_z);
     const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
@@ -456,7 +484,7 @@ __device__ __forceinline__ bool emit_rows_setup(EmitRows& E, const uint64_t* __r
                                                 int64_t blk)
 {
     E.lane = lane_id();
-    E.wv = threadIdx.x >> 6;
+    E.wv = wave_in_block();
     E.rl = row_lanes(g.W);
     const int64_t R = g.ny * g.nz;
     E.r0 = (blk * 4 + E.wv) * E.rl.rpw;
@@ -474,16 +502,15 @@ __device__ __forceinline__ bool emit_rows_setup(EmitRows& E, const uint64_t* __r
         if (__ballot(some) == 0ull) return false;
     }
     E.emit = E.ok && E.r >= g.emit_lo && E.r < g.emit_hi;
-    const int ny = (int)g.ny;
-    const int j = (int)(E.r % ny), k = (int)(E.r / ny);
-    E.has_y = j < ny - 1;
-    E.has_z = k < (int)g.nz - 1;
+    const RowJK jk = split_row(E.r0, E.rl.q, g.ny);
+    E.has_y = jk.j < g.ny - 1;
+    E.has_z = jk.k < g.nz - 1;
     E.L = load_lane_words(mask, g, E.r, E.rl.w, E.ok, E.has_y, E.has_z);
     E.v0 = voff[E.r0];
     E.t0 = toff[E.r0];
     E.vb = low_bits(g.nx - 64 * E.rl.w);
     E.eb = low_bits(g.nx - 1 - 64 * E.rl.w);
-    E.k0 = E.r0 / g.ny;
+    E.k0 = uniform_div(E.r0, g.ny);
     E.j0 = (int)(E.r0 - E.k0 * g.ny);
     return true;
 }
@@ -528,14 +555,14 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
     // against 1.69 ms); for shorter rows in output order, so that one step's stores are
     // contiguous (512^3 gyroid: 0.425 against 0.44 ms; 256^3 and 384^3 likewise).
     const bool word_major = rl.Wp >= 16;
+    const int wp_log2 = __builtin_ctz((unsigned)rl.Wp), rpw_log2 = 6 - wp_log2;
     const int tl = word_major ? rl.w * rl.rpw + rl.q : lane;                  // list position of this lane
-    const int from = word_major ? (lane % rl.rpw) * rl.Wp + lane / rl.rpw : lane;   // lane at position `lane`
+    const int from = word_major ? ((lane & (rl.rpw - 1)) << wp_log2) + (lane >> rpw_log2) : lane;   // lane at position `lane`
     const uint32_t c0t = (uint32_t)__shfl((int)c0, from), c12t = (uint32_t)__shfl((int)(c1 + c2), from);
     const uint32_t xincl_t = wave_incl_scan_u32(c0t), yzincl_t = wave_incl_scan_u32(c12t);
     const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl_t, 63);
     const uint32_t xfirst = (uint32_t)__shfl((int)(xincl_t - c0t), tl);
     const uint32_t yzfirst = (uint32_t)__shfl((int)(yzincl_t - c12t), tl);
-    const int wp_log2 = __builtin_ctz((unsigned)rl.Wp);
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp) {
         const uint32_t gtotal = grp ? total - xtotal : xtotal;

@@ -57,10 +57,12 @@ __global__ __launch_bounds__(256) void k_mc_count_rows(const uint64_t* __restric
          t += (int64_t)gridDim.x * blockDim.x)
         scan_state[t] = 0;
     const RowLanes rl = row_lanes(g.W);
-    const int64_t r = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * rl.rpw + rl.q;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave_in_block()) * rl.rpw;
+    const int64_t r = r0 + rl.q;
     const int64_t R = g.ny * g.nz;
     const bool ok = r < R && rl.w < g.W;
-    const int64_t j = r % g.ny, k = r / g.ny;
+    const RowJK jk = split_row(r0, rl.q, g.ny);
+    const int64_t j = jk.j, k = jk.k;
     const bool has_y = j < g.ny - 1, has_z = k < g.nz - 1;
     const LaneWords L = load_lane_words(mask, g, r, rl.w, ok, has_y && has_z, has_y && has_z);
     const CornerWords C = corner_words(L);
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
     s_first[threadIdx.x] = first_appearance(kCaseEdges[threadIdx.x]);
     __syncthreads();
     const int lane = lane_id();
-    const int wv = threadIdx.x >> 6;
+    const int wv = wave_in_block();
     McRowsLds& S = s_w[wv];
     const RowLanes rl = row_lanes(g.W);
     const int64_t R = g.ny * g.nz;
@@ -179,7 +181,8 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
     if (r0 >= R) return;
     const int64_t r = r0 + rl.q;
     const bool ok = r < R && rl.w < g.W;
-    const int64_t j = r % g.ny, k = r / g.ny;
+    const RowJK jk = split_row(r0, rl.q, g.ny);
+    const int64_t j = jk.j, k = jk.k;
     const bool cells = ok && j < g.ny - 1 && k < g.nz - 1;
     const LaneWords L = load_lane_words(mask, g, r, rl.w, cells, true, true);
     const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
@@ -224,7 +227,7 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
     const uint32_t clo[8] = {lo[0], lo[1], lo[3], lo[2], lo[4], lo[5], lo[7], lo[6]};
     const uint32_t chi[8] = {hi[0], hi[1], hi[3], hi[2], hi[4], hi[5], hi[7], hi[6]};
     const uint32_t cfirst = cincl - ncell;
-    const int64_t j0 = r0 % g.ny, k0 = r0 / g.ny;
+    const int64_t k0 = uniform_div(r0, g.ny), j0 = r0 - k0 * g.ny;
     uint64_t mid = EdgeOrder ? 0 : moff[r0];   // the wavefront's first Marching Cubes vertex
     for (uint32_t win = 0; win < ctotal; win += kMcCellCap) {
         const uint32_t wend = win + kMcCellCap;
