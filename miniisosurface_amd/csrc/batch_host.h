@@ -13,7 +13,10 @@ struct miso_batch {
     bool own_stream = false;
     int njobs = 0;
     std::vector<BatchJob> jobs;
-    std::vector<int64_t> cls_prefix, lane_prefix;    // work items per job, prefix sums
+    std::vector<int64_t> cls_prefix, lane_prefix;    // work items per classify group / per job, prefix sums
+    std::vector<int> group_job;                      // classify group q: jobs [group_job[q], group_job[q + 1])
+    int ngroups = 0;
+    int* d_group_job = nullptr;
     BatchJob* d_jobs = nullptr;
     int64_t* d_cls_prefix = nullptr;
     int64_t* d_lane_prefix = nullptr;
@@ -33,7 +36,7 @@ struct miso_batch {
     uint64_t cap_v = 0, cap_t = 0;
     int blocks_per_cu = 4, chunk = 4, num_cus = 256;
     int64_t cap_jobs = 0, cap_prefix_c = 0, cap_prefix_l = 0, cap_mask = 0, cap_counts = 0, cap_trims = 0;
-    int64_t cap_voff = 0, cap_toff = 0, cap_state = 0, cap_jobout = 0, cap_host = 0;
+    int64_t cap_voff = 0, cap_toff = 0, cap_state = 0, cap_jobout = 0, cap_host = 0, cap_groups = 0;
     bool counted = false;             // split-phase use: counts of the current job set are on the host
     hipGraphExec_t g_scan = nullptr, g_emit = nullptr, g_all = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -87,8 +90,8 @@ int enqueue_scan(miso_batch* b)
     hipLaunchKernelGGL(k_batch_zero, dim3((unsigned)std::min<int64_t>(1024, (b->state_words + 255) / 256)), dim3(256),
                        0, b->stream, b->d_state, b->state_words);
     hipLaunchKernelGGL(k_batch_classify, dim3(batch_grid(b, ncls)), dim3(256), 0, b->stream,
-                       (const BatchJob*)b->d_jobs, (const int64_t*)b->d_cls_prefix, b->njobs, ncls, b->chunk, b->d_mask,
-                       b->tickets() + 0);
+                       (const BatchJob*)b->d_jobs, (const int*)b->d_group_job, (const int64_t*)b->d_cls_prefix,
+                       b->ngroups, ncls, b->chunk, b->d_mask, b->tickets() + 0);
     hipLaunchKernelGGL(k_batch_count, dim3(batch_grid(b, nlane)), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
                        (const int64_t*)b->d_lane_prefix, b->njobs, nlane, b->chunk, (const uint64_t*)b->d_mask,
                        b->d_counts, b->d_trims, b->tickets() + 1);
@@ -179,6 +182,7 @@ void batch_release(miso_batch* b)
     destroy_graphs(b);
     dev_free(b->d_jobs);
     dev_free(b->d_cls_prefix);
+    dev_free(b->d_group_job);
     dev_free(b->d_lane_prefix);
     dev_free(b->d_mask);
     dev_free(b->d_counts);
@@ -233,7 +237,7 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
     if (njobs <= 0 || njobs > (1 << 24)) return fail(MISO_ERR_ARG, "job count %lld out of range", (long long)njobs);
     if (!vols || !isovals) return fail(MISO_ERR_ARG, "volumes or isovalues NULL");
     std::vector<BatchJob> jobs(njobs);
-    std::vector<int64_t> cls(njobs + 1, 0), lane(njobs + 1, 0);
+    std::vector<int64_t> lane(njobs + 1, 0);
     int64_t R = 0, mw = 0;
     for (int64_t j = 0; j < njobs; ++j) {
         const miso_volume* v = &vols[j];
@@ -254,8 +258,21 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
         R += rows;
         mw += rows * J.g.W;
         const int64_t per_block = 4 * geom_rows_per_wave(J.g);
-        cls[j + 1] = cls[j] + (rows + kRowsPerBlock - 1) / kRowsPerBlock;
         lane[j + 1] = lane[j] + (rows + per_block - 1) / per_block;
+    }
+    // classify groups: runs of consecutive jobs on one volume with the same dimensions (one
+    // volume at several isovalues), whose rows pass 1 then loads once for all of them
+    std::vector<int> grp{0};
+    std::vector<int64_t> cls{0};
+    for (int64_t j = 0; j < njobs;) {
+        const Geom& g = jobs[j].g;
+        int64_t e = j + 1;
+        while (e < njobs && e - j < kClassifyGroup && jobs[e].vol == jobs[j].vol && jobs[e].g.nx == g.nx &&
+               jobs[e].g.ny == g.ny && jobs[e].g.nz == g.nz)
+            ++e;
+        cls.push_back(cls.back() + (g.ny * g.nz + kRowsPerBlock - 1) / kRowsPerBlock);
+        grp.push_back((int)e);
+        j = e;
     }
     HIPCALL(hipStreamSynchronize(b->stream));   // the previous job set is no longer in use
     destroy_graphs(b);
@@ -263,6 +280,8 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
     b->njobs = (int)njobs;
     b->jobs.swap(jobs);
     b->cls_prefix.swap(cls);
+    b->group_job.swap(grp);
+    b->ngroups = (int)b->group_job.size() - 1;
     b->lane_prefix.swap(lane);
     b->R = R;
     b->mask_words = mw;
@@ -271,6 +290,7 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
     int rc = 0;
     if ((rc = grow(&b->d_jobs, &b->cap_jobs, njobs))) return rc;
     if ((rc = grow(&b->d_cls_prefix, &b->cap_prefix_c, njobs + 1))) return rc;
+    if ((rc = grow(&b->d_group_job, &b->cap_groups, njobs + 1))) return rc;
     if ((rc = grow(&b->d_lane_prefix, &b->cap_prefix_l, njobs + 1))) return rc;
     if ((rc = grow(&b->d_mask, &b->cap_mask, mw))) return rc;
     if ((rc = grow(&b->d_counts, &b->cap_counts, R))) return rc;
@@ -287,7 +307,9 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
         b->cap_host = njobs + 1;
     }
     HIPCALL(hipMemcpyAsync(b->d_jobs, b->jobs.data(), njobs * sizeof(BatchJob), hipMemcpyHostToDevice, b->stream));
-    HIPCALL(hipMemcpyAsync(b->d_cls_prefix, b->cls_prefix.data(), (njobs + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+    HIPCALL(hipMemcpyAsync(b->d_cls_prefix, b->cls_prefix.data(), (b->ngroups + 1) * sizeof(int64_t),
+                           hipMemcpyHostToDevice, b->stream));
+    HIPCALL(hipMemcpyAsync(b->d_group_job, b->group_job.data(), (b->ngroups + 1) * sizeof(int), hipMemcpyHostToDevice,
                            b->stream));
     HIPCALL(hipMemcpyAsync(b->d_lane_prefix, b->lane_prefix.data(), (njobs + 1) * sizeof(int64_t),
                            hipMemcpyHostToDevice, b->stream));

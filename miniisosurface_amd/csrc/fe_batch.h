@@ -18,6 +18,8 @@
 
 namespace miso {
 
+constexpr int kClassifyGroup = 16;   // jobs of one volume classified together (pass 1 of a batch)
+
 struct BatchJob {
     Geom g;
     const float* vol;
@@ -76,14 +78,25 @@ __global__ __launch_bounds__(256) void k_batch_zero(uint64_t* __restrict__ words
         words[t] = 0;
 }
 
+/* Pass 1 over groups of jobs: group q is jobs [group_job[q], group_job[q + 1]), consecutive jobs
+ * that read the same volume with the same dimensions (one volume at several isovalues); an item
+ * loads a block of its rows once and writes every job's sign-mask words. */
 __global__ __launch_bounds__(256) void k_batch_classify(const BatchJob* __restrict__ jobs,
-                                                        const int64_t* __restrict__ prefix, int njobs,
+                                                        const int* __restrict__ group_job,
+                                                        const int64_t* __restrict__ prefix, int ngroups,
                                                         int64_t nitems, int chunk, uint64_t* __restrict__ mask,
                                                         uint64_t* __restrict__ ticket)
 {
-    run_queue(ticket, prefix, njobs, nitems, chunk, [&](int j, int64_t blk) {
-        const BatchJob& J = jobs[j];
-        fe_classify_block(J.vol, J.g, J.iso, mask + J.mask_base, blk);
+    run_queue(ticket, prefix, ngroups, nitems, chunk, [&](int q, int64_t blk) {
+        const int j0 = group_job[q], n = group_job[q + 1] - j0;
+        const BatchJob& J = jobs[j0];
+        fe_classify_block_n(
+            J.vol, J.g, n,
+            [&](int t, float& iso, uint64_t*& m) {
+                iso = jobs[j0 + t].iso;
+                m = mask + jobs[j0 + t].mask_base;
+            },
+            blk);
     });
 }
 

@@ -33,70 +33,98 @@ constexpr uint64_t kValMask = (1ull << 62) - 1;
  * non-temporal load, so a wavefront step covers 256 voxels = 4 mask words); its 4 sign bits are placed at
  * bit 4 * (lane % 16) and OR-reduced over each 16-lane row with DPP, so lane 16 t + 15 ends up
  * holding mask word t of the step. Voxels past nx read as below the isovalue. Rows of at most
- * 256 voxels take one voxel per lane and __ballot instead (fewer instructions per short row). */
-__device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol, const Geom& g, float iso,
-                                                  uint64_t* __restrict__ mask, int64_t blk)
+ * 256 voxels take one voxel per lane and __ballot instead (fewer instructions per short row).
+ * The voxels are loaded once and classified against n isovalues of the same volume:
+ * sink(t, iso, mask) names isovalue t and the mask array it writes (n = 1 for one extraction). */
+template <class Sink>
+__device__ __forceinline__ void fe_classify_block_n(const float* __restrict__ vol, const Geom& g, int n, Sink sink,
+                                                    int64_t blk)
 {
     const int lane = lane_id();
     const int64_t r = blk * kRowsPerBlock + (threadIdx.x >> 6);
     if (r >= g.ny * g.nz) return;
     const float* __restrict__ row = vol + r * g.nx;
-    uint64_t* __restrict__ mrow = mask + r * g.W;
     if (g.W <= 4) {
         // short rows: one 4-byte voxel per lane and word, the sign bits straight from __ballot
         float v[4];
+        bool in[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int64_t i = 64 * u + lane;
-            v[u] = (u < g.W && i < g.nx) ? row[i] : 0.0f;
+            in[u] = u < g.W && i < g.nx;
+            v[u] = in[u] ? row[i] : 0.0f;
         }
+        for (int t = 0; t < n; ++t) {
+            float iso;
+            uint64_t* mask;
+            sink(t, iso, mask);
+            uint64_t* __restrict__ mrow = mask + r * g.W;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint64_t b = __ballot(64 * u + lane < g.nx && v[u] >= iso);
-            if (lane == 0 && u < g.W) mrow[u] = b;
+            for (int u = 0; u < 4; ++u) {
+                const uint64_t b = __ballot(in[u] && v[u] >= iso);
+                if (lane == 0 && u < g.W) mrow[u] = b;
+            }
         }
         return;
     }
     const int sub = lane & 15;
     constexpr int kSteps = 4;                      // 1024 voxels in flight per wavefront
     for (int64_t c0 = 0; c0 < g.nx; c0 += 256 * kSteps) {
-        uint32_t nib[kSteps];
+        float v[kSteps][4];
+        uint32_t in[kSteps];                       // bit c: voxel i + c lies in the row
 #pragma unroll
         for (int u = 0; u < kSteps; ++u) {
-            nib[u] = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[u][c] = 0.0f;
+            in[u] = 0;
             if (c0 + 256 * u >= g.nx) continue;    // wavefront-uniform: short rows skip whole steps
             const int64_t i = c0 + 256 * u + 4 * lane;
-            float v[4];
             if (i + 4 <= g.nx) {
                 const Vox4 q = load_vox4_stream(row + i);
-                v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+                v[u][0] = q.x; v[u][1] = q.y; v[u][2] = q.z; v[u][3] = q.w;
+                in[u] = 15u;
             } else {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) v[c] = i + c < g.nx ? row[i + c] : 0.0f;
+                for (int c = 0; c < 4; ++c) {
+                    v[u][c] = i + c < g.nx ? row[i + c] : 0.0f;
+                    in[u] |= (uint32_t)(i + c < g.nx) << c;
+                }
             }
-            uint32_t b = 0;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) b |= (uint32_t)(i + c < g.nx && v[c] >= iso) << c;
-            nib[u] = b;
         }
+        for (int t = 0; t < n; ++t) {
+            float iso;
+            uint64_t* mask;
+            sink(t, iso, mask);
+            uint64_t* __restrict__ mrow = mask + r * g.W;
 #pragma unroll
-        for (int u = 0; u < kSteps; ++u) {
-            if (c0 + 256 * u >= g.nx) break;
-            // bits of lane sub go to 4 sub .. 4 sub + 3 of the (lo, hi) 32-bit halves of the word
-            uint32_t lo = sub < 8 ? nib[u] << (4 * sub) : 0u;
-            uint32_t hi = sub < 8 ? 0u : nib[u] << (4 * (sub - 8));
-            lo |= dpp_u32<kDppRowShr1>(lo);
-            hi |= dpp_u32<kDppRowShr1>(hi);
-            lo |= dpp_u32<kDppRowShr2>(lo);
-            hi |= dpp_u32<kDppRowShr2>(hi);
-            lo |= dpp_u32<kDppRowShr4>(lo);
-            hi |= dpp_u32<kDppRowShr4>(hi);
-            lo |= dpp_u32<kDppRowShr8>(lo);
-            hi |= dpp_u32<kDppRowShr8>(hi);
-            const int64_t w = (c0 + 256 * u) / 64 + (lane >> 4);
-            if (sub == 15 && w < g.W) mrow[w] = (uint64_t)hi << 32 | lo;
+            for (int u = 0; u < kSteps; ++u) {
+                if (c0 + 256 * u >= g.nx) break;
+                uint32_t nib = 0;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) nib |= (uint32_t)(v[u][c] >= iso) << c;
+                nib &= in[u];
+                // bits of lane sub go to 4 sub .. 4 sub + 3 of the (lo, hi) 32-bit halves of the word
+                uint32_t lo = sub < 8 ? nib << (4 * sub) : 0u;
+                uint32_t hi = sub < 8 ? 0u : nib << (4 * (sub - 8));
+                lo |= dpp_u32<kDppRowShr1>(lo);
+                hi |= dpp_u32<kDppRowShr1>(hi);
+                lo |= dpp_u32<kDppRowShr2>(lo);
+                hi |= dpp_u32<kDppRowShr2>(hi);
+                lo |= dpp_u32<kDppRowShr4>(lo);
+                hi |= dpp_u32<kDppRowShr4>(hi);
+                lo |= dpp_u32<kDppRowShr8>(lo);
+                hi |= dpp_u32<kDppRowShr8>(hi);
+                const int64_t w = (c0 + 256 * u) / 64 + (lane >> 4);
+                if (sub == 15 && w < g.W) mrow[w] = (uint64_t)hi << 32 | lo;
+            }
         }
     }
+}
+
+__device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol, const Geom& g, float iso,
+                                                  uint64_t* __restrict__ mask, int64_t blk)
+{
+    fe_classify_block_n(vol, g, 1, [&](int, float& is, uint64_t*& m) { is = iso; m = mask; }, blk);
 }
 
 __global__ __launch_bounds__(256) void k_fe_classify(const float* __restrict__ vol, Geom g, float iso,

@@ -71,6 +71,32 @@ def test_batch_matches_single_volume():
     b.close()
 
 
+def test_batch_shared_volume_groups():
+    """Consecutive jobs on one volume are classified together, at most 16 to a group (the rows
+    loaded once for all of them): 20 isovalues of a volume with 300-voxel rows (16-byte loads and
+    a partial step), a job on another volume, 3 more isovalues of the first volume, and 5 of a
+    volume with 64-voxel rows (the __ballot path). Each job equals its single-volume run."""
+    import torch
+    from miniisosurface_amd import DeviceVolume, synth
+    from miniisosurface_amd.batch import BatchExtraction
+    a = synth.gaussians((300, 21, 9), count=4, seed=5, sigma_range=(3.0, 9.0))
+    g = synth.gyroid(40, 2)
+    c = synth.gaussians((64, 30, 17), count=3, seed=8, sigma_range=(3.0, 8.0))
+    ta, tg, tc = (torch.from_numpy(x).cuda() for x in (a, g, c))
+    jobs = [(ta, a, float(iso)) for iso in np.linspace(0.1, 0.9, 20)]
+    jobs.append((tg, g, 0.1))
+    jobs += [(ta, a, iso) for iso in (0.33, 0.5, 0.71)]
+    jobs += [(tc, c, float(iso)) for iso in np.linspace(0.2, 0.8, 5)]
+    print(f"[batch groups] {len(jobs)} jobs", flush=True)
+    refs = [_single(v, iso, (1, 1, 1), (0, 0, 0)) for _, v, iso in jobs]
+    assert sum(r.triangles.shape[0] for r in refs) > 0
+    b = BatchExtraction([(DeviceVolume.from_tensor(t), iso) for t, _, iso in jobs])
+    for rep in range(2):
+        b.run()
+        _check(b, refs)
+    b.close()
+
+
 def test_batch_against_oracle(oracle_mod):
     from miniisosurface_amd.batch import BatchExtraction
     jobs, tens = _jobs(seed=11)
