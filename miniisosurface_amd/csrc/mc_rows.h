@@ -249,8 +249,63 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
         }
         wave_sync();
         const uint32_t n = min((uint32_t)kMcCellCap, ctotal - win);
+        // The owned edges of consecutive 64-cell groups collect in S.verts until the next group
+        // might not fit: x-edges from the front, y/z-edges from the back, so that each step of
+        // the flush runs one vertex path on 64 lanes. Entry: window cell | edge << 9 | id << 13,
+        // the id relative to fbase (the first Marching Cubes vertex of the collection).
+        uint32_t fx = 0, fyz = 0;
+        uint64_t fbase = mid;
+        auto vertex_of = [&](uint32_t ve, bool xpath) {
+            const uint32_t cent = S.cells[ve & 511];
+            const int e = (int)((ve >> 9) & 15);
+            const int src = (int)(cent & 63), b = (int)((cent >> 6) & 63);
+            const int q = src >> __builtin_ctz((unsigned)rl.Wp), w = src & (rl.Wp - 1);
+            int64_t jj = j0 + q, kk = k0;
+            while (jj >= g.ny) {
+                jj -= g.ny;
+                ++kk;
+            }
+            const int64_t i = 64 * (int64_t)w + b;
+            float pp[3], nn[3];
+            if (Unit) {
+                // unit spacing, zero origin: the owner's corner coordinates are float(index),
+                // as Flying Edges' are, so the Flying Edges vertex code gives the same bits
+                const int ia = (int)i + edge_dx(e), ja = (int)jj + edge_dy(e), ka = (int)kk + edge_dz(e);
+                const float* __restrict__ v = vol + ((int64_t)ka * g.ny + ja) * g.nx + ia;
+                if (xpath)
+                    vertex_x<true>(v, g, iso, ia, ja, ka, pp, nn);
+                else
+                    vertex_yz<true>(v, g, iso, edge_axis(e), ia, ja, ka, pp, nn);
+            } else {
+                float A[3], B[3];
+                mc_corners(g, i, jj, kk, e, A, B);
+                edge_point(vol, g, iso, edge_axis(e), i + edge_dx(e), jj + edge_dy(e), kk + edge_dz(e), A, B, pp, nn);
+            }
+            // the edge's Flying Edges id, as the triangle pass computes it
+            const int cq = combo_of_edge(e);
+            const int p = b + edge_dx(e);
+            const uint64_t cwq = S.cut[cq][src];
+            const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
+            const uint64_t fe = v0 + S.base[cq][src] + (uint32_t)__popcll(below);
+            const uint64_t id = EdgeOrder ? (uint64_t)fe_to_mc[fe] : fbase + (ve >> 13);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                pts[3 * id + c] = pp[c];
+                nrm[3 * id + c] = nn[c];
+            }
+            if (!EdgeOrder) fe_to_mc[fe] = (Index)id;
+        };
+        auto flush = [&]() {
+            wave_sync();
+            for (uint32_t v = lane; v < fx; v += 64) vertex_of(S.verts[v], true);
+            for (uint32_t v = lane; v < fyz; v += 64) vertex_of(S.verts[kMcVertCap - 1 - v], false);
+            wave_sync();
+            fx = fyz = 0;
+            fbase = mid;
+        };
+        constexpr uint32_t kXEdges = 0x55u;   // edges 0, 2, 4, 6 run along x
         for (uint32_t g0 = 0; g0 < n; g0 += 64) {
-            // one cell per lane: its owned cut edges, in first-appearance order, into the list
+            // one cell per lane: its owned cut edges, in first-appearance order
             const uint32_t idx = g0 + (uint32_t)lane;
             uint32_t ent = 0, own = 0;
             uint64_t order = 0;
@@ -266,66 +321,37 @@ __global__ __launch_bounds__(256) void k_mc_vertices_rows(const float* __restric
                 order = s_first[ent >> 12];
                 own = (uint32_t)(order >> 48) & 0xFFFu & mc_owned_edges(w == 0 && b == 0, jj == 0, kk == 0);
             }
-            const uint32_t no = (uint32_t)__popc(own);
-            const uint32_t vincl = wave_incl_scan_u32(no);
-            const uint32_t vtot = (uint32_t)__builtin_amdgcn_readlane((int)vincl, 63);
+            const uint32_t no = (uint32_t)__popc(own), nox = (uint32_t)__popc(own & kXEdges);
+            // both counts in one scan (a group has at most 64 * 12 owned edges)
+            const uint32_t inc = wave_incl_scan_u32(no | nox << 16);
+            const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
+            const uint32_t vtot = tot & 0xFFFFu, xtot = tot >> 16;
+            if (fx + fyz + vtot > (uint32_t)kMcVertCap) flush();
             if (own) {
-                // owned edges in first-appearance order: rank = owned edges appearing earlier
+                // first-visit rank of an owned edge: owned edges appearing earlier in the cell's
+                // triangles, after the vertices of earlier lanes and groups
                 uint32_t posmask = 0;
                 for (uint32_t o = own; o; o &= o - 1) posmask |= 1u << ((order >> (4 * __builtin_ctz(o))) & 0xF);
-                const uint32_t base = vincl - no;
+                const uint32_t base = (uint32_t)(mid - fbase) + (inc & 0xFFFFu) - no;
+                uint32_t sx = fx + (inc >> 16) - nox;
+                uint32_t syz = fyz + ((inc & 0xFFFFu) - no) - ((inc >> 16) - nox);
                 for (uint32_t o = own; o; o &= o - 1) {
                     const int e = __builtin_ctz(o);
                     const uint32_t p = (uint32_t)(order >> (4 * e)) & 0xF;
-                    S.verts[base + (uint32_t)__popc(posmask & ((1u << p) - 1u))] = (uint32_t)lane | (uint32_t)e << 6;
-                }
-            }
-            wave_sync();
-            for (uint32_t v = lane; v < vtot; v += 64) {
-                const uint32_t ve = S.verts[v];
-                const uint32_t cent = S.cells[g0 + (ve & 63)];
-                const int e = (int)(ve >> 6);
-                const int src = (int)(cent & 63), b = (int)((cent >> 6) & 63);
-                const int q = src >> __builtin_ctz((unsigned)rl.Wp), w = src & (rl.Wp - 1);
-                int64_t jj = j0 + q, kk = k0;
-                while (jj >= g.ny) {
-                    jj -= g.ny;
-                    ++kk;
-                }
-                const int64_t i = 64 * (int64_t)w + b;
-                float pp[3], nn[3];
-                if (Unit) {
-                    // unit spacing, zero origin: the owner's corner coordinates are float(index),
-                    // as Flying Edges' are, so the Flying Edges vertex code gives the same bits
-                    const int ia = (int)i + edge_dx(e), ja = (int)jj + edge_dy(e), ka = (int)kk + edge_dz(e);
-                    const float* __restrict__ v = vol + ((int64_t)ka * g.ny + ja) * g.nx + ia;
-                    if (edge_axis(e) == 0)
-                        vertex_x<true>(v, g, iso, ia, ja, ka, pp, nn);
+                    const uint32_t rel = base + (uint32_t)__popc(posmask & ((1u << p) - 1u));
+                    const uint32_t ve = idx | (uint32_t)e << 9 | rel << 13;
+                    if ((kXEdges >> e) & 1u)
+                        S.verts[sx++] = ve;
                     else
-                        vertex_yz<true>(v, g, iso, edge_axis(e), ia, ja, ka, pp, nn);
-                } else {
-                    float A[3], B[3];
-                    mc_corners(g, i, jj, kk, e, A, B);
-                    edge_point(vol, g, iso, edge_axis(e), i + edge_dx(e), jj + edge_dy(e), kk + edge_dz(e), A, B, pp,
-                               nn);
+                        S.verts[kMcVertCap - 1 - syz++] = ve;
                 }
-                // the edge's Flying Edges id, as the triangle pass computes it
-                const int cq = combo_of_edge(e);
-                const int p = b + edge_dx(e);
-                const uint64_t cwq = S.cut[cq][src];
-                const uint64_t below = p == 64 ? cwq : (cwq & ((1ull << p) - 1ull));
-                const uint64_t fe = v0 + S.base[cq][src] + (uint32_t)__popcll(below);
-                const uint64_t id = EdgeOrder ? (uint64_t)fe_to_mc[fe] : mid + v;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    pts[3 * id + c] = pp[c];
-                    nrm[3 * id + c] = nn[c];
-                }
-                if (!EdgeOrder) fe_to_mc[fe] = (Index)id;
             }
+            fx += xtot;
+            fyz += vtot - xtot;
             mid += vtot;
-            wave_sync();
         }
+        flush();
+        wave_sync();
     }
 }
 
