@@ -235,7 +235,13 @@ __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__
     // positions as unsigned: "none" is 0xFFFFFFFF for the minimum and 0 (= last + 1) for the maximum
     uint32_t first = trim ? (uint32_t)(64 * rl.w + __builtin_ctzll(trim)) : 0xFFFFFFFFu;
     uint32_t lastp1 = trim ? (uint32_t)(64 * rl.w + 64 - __builtin_clzll(trim)) : 0u;
-    uint32_t tc = (has_y && has_z) ? word_tri_count(corner_words(L), eb) : 0u;
+    // word_tri_count is 0 where every corner of the lane's cells is below the isovalue: the 4
+    // words and bit 0 of the 4 next words (the corners i + 1 of bit 63) all zero. A wavefront
+    // with only such lanes skips it (most of them at a sparse, high isovalue).
+    const uint64_t any = L.m[0] | L.m[1] | L.m[2] | L.m[3] | ((L.n[0] | L.n[1] | L.n[2] | L.n[3]) & 1ull);
+    const bool live = has_y && has_z && any != 0ull;
+    uint32_t tc = 0u;
+    if (__ballot(live) != 0ull) tc = (has_y && has_z) ? word_tri_count(corner_words(L), eb) : 0u;
     // two counts per 32-bit scan: row totals stay below 2^16 (nx <= 4096, <= 5 triangles a cell)
     const uint32_t xy = seg_sum((uint32_t)__popcll(xcut) | (uint32_t)__popcll(ycut) << 16, rl.Wp);
     const uint32_t zt = seg_sum((uint32_t)__popcll(zcut) | tc << 16, rl.Wp);

@@ -77,6 +77,12 @@ def _cases():
     # whole tile rows; 16-row blocks, tiles of 2 blocks, every block in a tile row
     yield "gauss_64x2048x7", synth.gaussians((64, 2048, 7), 4, 31, sigma_range=(3.0, 9.0)), 0.35, (1, 1, 1), (0, 0, 0)
     yield "rand_1024x256x9_sp", rng.uniform(0, 1, (9, 256, 1024)).astype(np.float32), 0.5, (0.5, 1.5, 1.0), (0.0, 1.0, -2.0)
+    # empty but for isolated voxels at word boundaries (i = 63, 64, 127, 128, 129): pass 2 skips
+    # the triangle count of wavefronts whose words are all below the isovalue
+    iso_vox = np.zeros((7, 40, 130), np.float32)
+    for (k, j, i) in [(1, 3, 63), (2, 17, 64), (3, 30, 127), (4, 31, 128), (5, 38, 129), (0, 0, 64), (6, 39, 0)]:
+        iso_vox[k, j, i] = 1.0
+    yield "isolated_130x40x7", iso_vox, 0.5, (1, 1, 1), (0, 0, 0)
     yield "checker_64", (np.indices((64, 64, 64)).sum(0) % 2).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
     yield "gauss_96x80x72", synth.gaussians((96, 80, 72), 6, 3, sigma_range=(5.0, 14.0)), 0.3, (1, 1, 1), (0, 0, 0)
     yield "gyroid_128", synth.gyroid(128, 1), 0.0, (1, 1, 1), (0, 0, 0)
