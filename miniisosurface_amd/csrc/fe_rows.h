@@ -227,6 +227,17 @@ __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__
     const int64_t j = jk.j, k = jk.k;
     const bool has_y = j < g.ny - 1, has_z = k < g.nz - 1;
     const LaneWords L = load_lane_words(mask, g, r, rl.w, ok, has_y, has_z);
+    // the words and bit 0 of the next words (corner i + 1 of bit 63), as far as the lane loaded them
+    const uint64_t any = L.m[0] | L.m[1] | L.m[2] | L.m[3] | ((L.n[0] | L.n[1] | L.n[2] | L.n[3]) & 1ull);
+    if (__ballot(any != 0ull) == 0ull) {
+        // every voxel the wavefront's rows see is below the isovalue (most wavefronts at a
+        // sparse, high isovalue): no cut edge, no triangle, an empty trim
+        if (r < R && rl.w == 0) {
+            counts[r] = RowCounts{0u, 0u, 0u, 0u};
+            trims[r] = RowTrim{(uint32_t)g.nx, 0u};
+        }
+        return;
+    }
     const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
     const uint64_t xcut = (L.m[0] ^ shifted(L.m[0], L.n[0])) & eb;
     const uint64_t ycut = has_y ? ((L.m[0] ^ L.m[1]) & vb) : 0;
@@ -235,10 +246,8 @@ __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__
     // positions as unsigned: "none" is 0xFFFFFFFF for the minimum and 0 (= last + 1) for the maximum
     uint32_t first = trim ? (uint32_t)(64 * rl.w + __builtin_ctzll(trim)) : 0xFFFFFFFFu;
     uint32_t lastp1 = trim ? (uint32_t)(64 * rl.w + 64 - __builtin_clzll(trim)) : 0u;
-    // word_tri_count is 0 where every corner of the lane's cells is below the isovalue: the 4
-    // words and bit 0 of the 4 next words (the corners i + 1 of bit 63) all zero. A wavefront
-    // with only such lanes skips it (most of them at a sparse, high isovalue).
-    const uint64_t any = L.m[0] | L.m[1] | L.m[2] | L.m[3] | ((L.n[0] | L.n[1] | L.n[2] | L.n[3]) & 1ull);
+    // word_tri_count is 0 where every corner of the lane's cells is below the isovalue (`any`
+    // zero); a wavefront with only such lanes skips it
     const bool live = has_y && has_z && any != 0ull;
     uint32_t tc = 0u;
     if (__ballot(live) != 0ull) tc = (has_y && has_z) ? word_tri_count(corner_words(L), eb) : 0u;
