@@ -16,8 +16,8 @@ def pytest_configure(config):
 
 
 def pytest_collection_modifyitems(config, items):
-    # GPU tests fail loudly when no GPU is present: a test marked gpu is only ever selected with
-    # -m gpu on a GPU box. Nothing here skips them silently.
+    # GPU tests fail loudly when no GPU is present: a test marked gpu is meant to be selected
+    # with -m gpu on a machine with an MI355X. Nothing here skips them silently.
     pass
 
 
