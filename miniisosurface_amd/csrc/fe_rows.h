@@ -564,19 +564,22 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
     const uint32_t pos2 = before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w);
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     // Two lists, x-edges and then y/z-edges, so that the lanes of one step take the same load
-    // pattern; each entry carries its output position. For rows of 16 words and more, entries
-    // run word-major (word w of all the wavefront's rows, then word w + 1), so that one step's
-    // gathers fall in a narrow x-range of neighbouring rows and share cache lines (1024^3: 1.65
-    // against 1.69 ms); for shorter rows in output order, so that one step's stores are
-    // contiguous (512^3 gyroid: 0.425 against 0.44 ms; 256^3 and 384^3 likewise).
+    // pattern; each entry carries its output position. The x-edges are listed in lane order,
+    // which is output order within each row, so that a step stores runs of consecutive vertices
+    // (they mostly fit one step, whose gathers then touch the same lines in any order; 1024^3:
+    // 1.594 against 1.615 ms). For rows of 16 words and more, the y/z-edges run word-major (word
+    // w of all the wavefront's rows, then word w + 1), so that each of their steps gathers from a
+    // narrow x-range of neighbouring rows and shares cache lines (1024^3: 1.65 against 1.69 ms);
+    // for shorter rows in output order, so that one step's stores are contiguous (512^3 gyroid:
+    // 0.425 against 0.44 ms; 256^3 and 384^3 likewise).
     const bool word_major = rl.Wp >= 16;
     const int wp_log2 = __builtin_ctz((unsigned)rl.Wp), rpw_log2 = 6 - wp_log2;
-    const int tl = word_major ? rl.w * rl.rpw + rl.q : lane;                  // list position of this lane
+    const int tl = word_major ? rl.w * rl.rpw + rl.q : lane;                  // y/z list position of this lane
     const int from = word_major ? ((lane & (rl.rpw - 1)) << wp_log2) + (lane >> rpw_log2) : lane;   // lane at position `lane`
-    const uint32_t c0t = (uint32_t)__shfl((int)c0, from), c12t = (uint32_t)__shfl((int)(c1 + c2), from);
-    const uint32_t xincl_t = wave_incl_scan_u32(c0t), yzincl_t = wave_incl_scan_u32(c12t);
-    const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl_t, 63);
-    const uint32_t xfirst = (uint32_t)__shfl((int)(xincl_t - c0t), tl);
+    const uint32_t c12t = (uint32_t)__shfl((int)(c1 + c2), from);
+    const uint32_t xincl = wave_incl_scan_u32(c0), yzincl_t = wave_incl_scan_u32(c12t);
+    const uint32_t xtotal = (uint32_t)__builtin_amdgcn_readlane((int)xincl, 63);
+    const uint32_t xfirst = xincl - c0;
     const uint32_t yzfirst = (uint32_t)__shfl((int)(yzincl_t - c12t), tl);
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp) {
