@@ -136,6 +136,16 @@ class FlyingEdgesAlgorithm:
         check(lib().miso_fe_counts(self._h, ctypes.byref(v), ctypes.byref(t)), "counts")
         return v.value, t.value
 
+    def set_timing(self, passes=(1, 2, 3, 4)):
+        """Time only these passes with HIP events in later runs (the others report 0); each
+        timed pass adds an event pair between the kernels."""
+        mask = 0
+        for p in passes:
+            if p not in (1, 2, 3, 4):
+                raise ValueError("passes are 1..4")
+            mask |= 1 << (p - 1)
+        check(lib().miso_fe_set_timing(self._h, ctypes.c_uint(mask)), "set_timing")
+
     def pass_times_ms(self):
         ms = (ctypes.c_float * 4)()
         check(lib().miso_fe_pass_times(self._h, ms), "pass_times")
@@ -181,6 +191,7 @@ class FlyingEdgesAlgorithm:
 
 def flying_edges(image, isoval, stream=None) -> TriangleMesh:
     algo = FlyingEdgesAlgorithm(image, isoval, stream)
+    algo.set_timing(())   # no pass times are returned: no event pairs between the passes
     algo.run()
     return algo.moveOutput()
 

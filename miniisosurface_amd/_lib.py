@@ -63,6 +63,7 @@ def lib():
         "miso_fe_device_output": [P, PP, PP, PP, ctypes.POINTER(I)],
         "miso_fe_set_index_bits": [P, I],
         "miso_fe_pass_times": [P, P],
+        "miso_fe_set_timing": [P, ctypes.c_uint],
         "miso_fe_rows": [P, P, P, P, P, P, P, P, P],
         "miso_fe_cube_cases": [P, P, I],
         "miso_fe_slabs_create": [ctypes.POINTER(Volume), F, ctypes.c_int64, P, PP],

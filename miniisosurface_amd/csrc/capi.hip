@@ -500,6 +500,7 @@ struct miso_fe {
     int done = 0;                    // last completed pass
     hipEvent_t ev[8] = {};           // start/stop of passes 1..4
     bool events = false;
+    unsigned timed = 0xFu;           // bit p - 1: pass p's device time is recorded with events
     float pass_ms[4] = {0, 0, 0, 0};
 };
 
@@ -511,7 +512,8 @@ int fe_pass_launch(miso_fe* h, int pass, const uint64_t* need = nullptr)
 {
     RowState& s = h->st;
     if (!has_cells(s.g)) return MISO_OK;
-    HIPCALL(hipEventRecord(h->ev[2 * (pass - 1)], s.stream));
+    const bool timed = pass >= 1 && pass <= 4 && ((h->timed >> (pass - 1)) & 1u);
+    if (timed) HIPCALL(hipEventRecord(h->ev[2 * (pass - 1)], s.stream));
     switch (pass) {
     case 1:
         if (int rc = s.launch_classify(h->iso)) return rc;
@@ -560,7 +562,7 @@ int fe_pass_launch(miso_fe* h, int pass, const uint64_t* need = nullptr)
     default:
         return fail(MISO_ERR_ARG, "pass must be 1..4");
     }
-    HIPCALL(hipEventRecord(h->ev[2 * (pass - 1) + 1], s.stream));
+    if (timed) HIPCALL(hipEventRecord(h->ev[2 * (pass - 1) + 1], s.stream));
     return MISO_OK;
 }
 
@@ -579,7 +581,10 @@ int fe_after_scan(miso_fe* h)
 int fe_collect_times(miso_fe* h)
 {
     if (!has_cells(h->st.g)) return MISO_OK;
-    for (int p = 0; p < 4; ++p) HIPCALL(hipEventElapsedTime(&h->pass_ms[p], h->ev[2 * p], h->ev[2 * p + 1]));
+    for (int p = 0; p < 4; ++p) {
+        h->pass_ms[p] = 0.0f;
+        if ((h->timed >> p) & 1u) HIPCALL(hipEventElapsedTime(&h->pass_ms[p], h->ev[2 * p], h->ev[2 * p + 1]));
+    }
     return MISO_OK;
 }
 
@@ -693,6 +698,14 @@ extern "C" int miso_fe_set_index_bits(miso_fe* h, int index_bits)
     if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
     if (index_bits != 32 && index_bits != 64) return fail(MISO_ERR_ARG, "index_bits must be 32 or 64");
     h->out.force64 = index_bits == 64;
+    return MISO_OK;
+}
+
+extern "C" int miso_fe_set_timing(miso_fe* h, unsigned passes)
+{
+    if (!h) return fail(MISO_ERR_ARG, "handle is NULL");
+    if (passes > 0xFu) return fail(MISO_ERR_ARG, "passes is a mask of bits 0..3");
+    h->timed = passes;
     return MISO_OK;
 }
 

@@ -51,8 +51,32 @@ def test_calls_out_of_order_and_bad_values():
     assert L.miso_fe_pass(h, 3) == 0 and L.miso_fe_pass(h, 4) == 0
     assert L.miso_fe_counts(h, ctypes.byref(v), ctypes.byref(t)) == 0 and v.value > 0
     assert L.miso_fe_run(h) == 0                            # still usable
+    assert L.miso_fe_set_timing(h, 0x10) == -3 and L.miso_fe_set_timing(None, 1) == -3
     L.miso_fe_destroy(h)
     L.miso_fe_destroy(None)                                 # no-op
+
+
+def test_pass_timing_selection():
+    """miso_fe_set_timing: only the selected passes report a device time, the others 0, and the
+    mesh does not depend on it."""
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+    from helpers import bits_equal
+    vol = synth.gaussians(48, 3, 1, sigma_range=(4.0, 10.0))
+    meshes = []
+    for passes in [(1, 2, 3, 4), (4,), (), (1, 3)]:
+        a = m.FlyingEdgesAlgorithm(m.Image3D(vol), 0.5)
+        a.set_timing(passes)
+        a.run()
+        ms = a.pass_times_ms()
+        for p in range(1, 5):
+            assert (ms[p - 1] > 0.0) == (p in passes), (passes, ms)
+        meshes.append(a.moveOutput())
+    for mesh in meshes[1:]:
+        assert bits_equal(mesh.points, meshes[0].points) and bits_equal(mesh.triangles, meshes[0].triangles)
+    a = m.FlyingEdgesAlgorithm(m.Image3D(vol), 0.5)
+    with pytest.raises(ValueError):
+        a.set_timing((0,))
 
 
 def test_batch_argument_errors():
