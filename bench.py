@@ -329,6 +329,9 @@ def bench_batched(reps):
 
     def run_set(name, jobs, algos, set_iso=None):
         # one extraction at a time: each runs its four passes with its own launches and sync
+        # (no pass timing, so no event pairs between its kernels)
+        for a in algos:
+            a.set_timing(())
         for a in algos[:2]:
             a.run()
         torch.cuda.synchronize()

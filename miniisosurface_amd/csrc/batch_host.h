@@ -270,7 +270,8 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
         while (e < njobs && e - j < kClassifyGroup && jobs[e].vol == jobs[j].vol && jobs[e].g.nx == g.nx &&
                jobs[e].g.ny == g.ny && jobs[e].g.nz == g.nz)
             ++e;
-        cls.push_back(cls.back() + (g.ny * g.nz + kRowsPerBlock - 1) / kRowsPerBlock);
+        const int64_t per_item = batch_classify_rows(g);
+        cls.push_back(cls.back() + (g.ny * g.nz + per_item - 1) / per_item);
         grp.push_back((int)e);
         j = e;
     }

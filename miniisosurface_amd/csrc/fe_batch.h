@@ -20,6 +20,9 @@ namespace miso {
 
 constexpr int kClassifyGroup = 16;   // jobs of one volume classified together (pass 1 of a batch)
 
+/* rows per pass 1 work item of a batch job (k_batch_classify) */
+inline int64_t batch_classify_rows(const Geom& g) { return g.W <= 4 ? 4 * kShortRows : kRowsPerBlock; }
+
 struct BatchJob {
     Geom g;
     const float* vol;
@@ -90,13 +93,15 @@ __global__ __launch_bounds__(256) void k_batch_classify(const BatchJob* __restri
     run_queue(ticket, prefix, ngroups, nitems, chunk, [&](int q, int64_t blk) {
         const int j0 = group_job[q], n = group_job[q + 1] - j0;
         const BatchJob& J = jobs[j0];
-        fe_classify_block_n(
-            J.vol, J.g, n,
-            [&](int t, float& iso, uint64_t*& m) {
-                iso = jobs[j0 + t].iso;
-                m = mask + jobs[j0 + t].mask_base;
-            },
-            blk);
+        auto sink = [&](int t, float& iso, uint64_t*& m) {
+            iso = jobs[j0 + t].iso;
+            m = mask + jobs[j0 + t].mask_base;
+        };
+        // an item is 4 kShortRows rows of a short-row job (batch_classify_rows), else 4 rows
+        if (J.g.W <= 4)
+            fe_classify_short_rows_n(J.vol, J.g, n, sink, blk);
+        else
+            fe_classify_block_n(J.vol, J.g, n, sink, blk);
     });
 }
 

@@ -121,6 +121,48 @@ __device__ __forceinline__ void fe_classify_block_n(const float* __restrict__ vo
     }
 }
 
+/* Rows of at most 256 voxels (W <= 4) in blocks of 4 kShortRows rows: each wavefront loads
+ * kShortRows consecutive rows (one voxel per lane and word, as the short-row path above) before
+ * classifying them, one memory round trip for all of them. The batch's pass 1 takes this for
+ * its short-row jobs, where a work item otherwise moves one short row per wavefront. */
+constexpr int kShortRows = 4;
+
+template <class Sink>
+__device__ __forceinline__ void fe_classify_short_rows_n(const float* __restrict__ vol, const Geom& g, int n,
+                                                         Sink sink, int64_t blk)
+{
+    const int lane = lane_id();
+    const int64_t R = g.ny * g.nz;
+    const int64_t r0 = (blk * 4 + (threadIdx.x >> 6)) * kShortRows;
+    if (r0 >= R) return;
+    float v[kShortRows][4];
+    bool in[kShortRows][4];
+#pragma unroll
+    for (int q = 0; q < kShortRows; ++q) {
+        const int64_t r = r0 + q;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t i = 64 * u + lane;
+            in[q][u] = r < R && u < g.W && i < g.nx;
+            v[q][u] = in[q][u] ? vol[r * g.nx + i] : 0.0f;
+        }
+    }
+    for (int t = 0; t < n; ++t) {
+        float iso;
+        uint64_t* mask;
+        sink(t, iso, mask);
+#pragma unroll
+        for (int q = 0; q < kShortRows; ++q) {
+            const int64_t r = r0 + q;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint64_t b = __ballot(in[q][u] && v[q][u] >= iso);
+                if (lane == 0 && u < g.W && r < R) mask[r * g.W + u] = b;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void fe_classify_block(const float* __restrict__ vol, const Geom& g, float iso,
                                                   uint64_t* __restrict__ mask, int64_t blk)
 {
