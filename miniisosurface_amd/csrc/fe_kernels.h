@@ -22,7 +22,7 @@ namespace miso {
 
 constexpr int kRowsPerBlock = 4;        // one wavefront per grid row, 4 waves per block
 constexpr int kSegWords = 16;           // pass 4 processes a row in segments of 16 words (1024 voxels)
-constexpr int kScanItems = 4;
+constexpr int kScanItems = 16;
 constexpr int kScanTile = 256 * kScanItems;
 constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagPre = 2ull << 62;
