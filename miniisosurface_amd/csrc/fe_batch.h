@@ -5,8 +5,10 @@
  * The jobs' grid rows are concatenated into one row space: the per-row arrays (sign masks,
  * counts, trims, offsets) of job j start at row_base / mask_base. Each pass is one persistent
  * kernel whose blocks take chunks of work items from a device ticket counter; an item is one
- * block of the single-volume kernels (4 grid rows for classification, 4 wavefronts of lane rows
- * for counting and emission) of one job, found by a binary search of the jobs' item prefix.
+ * block of the single-volume kernels (4 wavefronts of lane rows for counting and emission) of
+ * one job, found by a binary search of the jobs' item prefix. Classification works on groups of
+ * consecutive jobs that read the same volume (one volume at several isovalues): an item is 4
+ * grid rows of the group (16 when rows hold at most 256 voxels), loaded once for all its jobs.
  * Pass 3 is the single-volume scan over the concatenated rows, so job j's output is the
  * contiguous range [voff[row_base_j], voff[row_base_{j+1}]) of one shared buffer; its triangles
  * index its own vertices from 0.
