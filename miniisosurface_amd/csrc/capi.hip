@@ -277,16 +277,29 @@ struct McState {
         d_map = nullptr;
     }
 
+    /* first-visit order with the row-lane kernels: the owned-edge counts come with pass 2
+     * (k_fe_mc_count_rows), launched by launch_fe_count */
+    bool fused_count(const RowState& s) const { return order != MISO_MC_ORDER_EDGE && s.lane_rows(); }
+
+    /* pass 2 of the Flying Edges row state, with the owned-edge counts where fused_count */
+    int launch_fe_count(RowState& s)
+    {
+        if (!fused_count(s)) return s.launch_count();
+        hipLaunchKernelGGL(k_fe_mc_count_rows, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                           (const uint64_t*)s.d_mask, s.g, s.d_counts, s.d_trims, s.d_scan,
+                           (int64_t)(1 + 2 * s.ntiles), d_owned, d_scan, (int64_t)(1 + ntiles));
+        HIPCALL(hipGetLastError());
+        return MISO_OK;
+    }
+
     int launch_count(RowState& s)
     {
         if (order == MISO_MC_ORDER_EDGE) return MISO_OK;   // no first-visit offsets needed
-        if (s.lane_rows())
-            hipLaunchKernelGGL(k_mc_count_rows, dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
-                               (const uint64_t*)s.d_mask, s.g, d_owned, d_scan, (int64_t)(1 + ntiles));
-        else
+        if (!fused_count(s)) {
             hipLaunchKernelGGL(k_mc_count, dim3(s.row_blocks()), dim3(256), 0, s.stream, (const uint64_t*)s.d_mask,
                                s.g, (const RowCounts*)s.d_counts, d_owned, d_scan, (int64_t)(1 + ntiles));
-        HIPCALL(hipGetLastError());
+            HIPCALL(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_scan_u32, dim3((unsigned)ntiles), dim3(256), 0, s.stream, (const uint32_t*)d_owned,
                            s.g.ny * s.g.nz, d_moff, d_scan, ntiles, d_total);
         HIPCALL(hipGetLastError());
@@ -1110,7 +1123,7 @@ extern "C" int miso_mc_run(miso_mc* h)
     }
     HIPCALL(hipEventRecord(h->ev[0], s.stream));
     if (int rc = s.launch_classify(h->iso)) return rc;
-    if (int rc = s.launch_count()) return rc;
+    if (int rc = h->mc.launch_fe_count(s)) return rc;
     if (int rc = s.launch_scan()) return rc;  // FE row offsets: the rank -> id map layout
     if (int rc = h->mc.launch_count(s)) return rc;
     HIPCALL(hipStreamSynchronize(s.stream));

@@ -213,10 +213,21 @@ __device__ __forceinline__ uint32_t word_tri_count(const CornerWords& C, uint64_
 }
 
 /* ------------------------------------------------------------------ pass 1b + 2 */
+/* per-row work beside the counts (none here; first-visit Marching Cubes adds its owned-edge
+ * counts, k_fe_mc_count_rows): called by every lane with the lane's words, row and flags, and
+ * `empty` when every word the wavefront loaded is zero */
+struct NoRowExtra {
+    __device__ void operator()(const LaneWords&, const RowLanes&, int64_t, bool, bool, bool, int64_t, int64_t,
+                               bool) const
+    {
+    }
+};
+
 /* the 4 wavefronts of lane-row block blk (rows 4 rpw blk ..) */
+template <class Extra = NoRowExtra>
 __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__ mask, const Geom& g,
                                                     RowCounts* __restrict__ counts, RowTrim* __restrict__ trims,
-                                                    int64_t blk)
+                                                    int64_t blk, Extra extra = Extra())
 {
     const RowLanes rl = row_lanes(g.W);
     const int64_t r0 = (blk * 4 + wave_in_block()) * rl.rpw;
@@ -236,6 +247,7 @@ __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__
             counts[r] = RowCounts{0u, 0u, 0u, 0u};
             trims[r] = RowTrim{(uint32_t)g.nx, 0u};
         }
+        extra(L, rl, r, ok, has_y, has_z, j, k, true);
         return;
     }
     const uint64_t vb = low_bits(g.nx - 64 * rl.w), eb = low_bits(g.nx - 1 - 64 * rl.w);
@@ -262,6 +274,7 @@ __device__ __forceinline__ void fe_count_rows_block(const uint64_t* __restrict__
         counts[r] = RowCounts{xc, yc, zc, tc};
         trims[r] = RowTrim{lastp1 == 0 ? (uint32_t)g.nx : first, lastp1};
     }
+    extra(L, rl, r, ok, has_y, has_z, j, k, false);
 }
 
 __global__ __launch_bounds__(256) void k_fe_count_rows(const uint64_t* __restrict__ mask, Geom g,

@@ -8,7 +8,8 @@
  * cell's case triangle list. Triangles are the Flying Edges triangle stream with each vertex id
  * mapped to its Marching Cubes id.
  *
- *   k_mc_count_rows    owned cut edges per cell row, bit-sliced over the 64 cells of a word
+ *   k_fe_mc_count_rows Flying Edges pass 2 and the owned cut edges per cell row, bit-sliced over
+ *                      the 64 cells of a word, in one pass over the mask words
  *   k_mc_vertices_rows owners' points and normals at their Marching Cubes ids, and the map
  *                      from Flying Edges vertex id to Marching Cubes id
  *   k_fe_emit_rows with a map (fe_rows.h) writes the triangles
@@ -49,22 +50,12 @@ __device__ __forceinline__ uint64_t corner_plane(const CornerWords& C, int q)
     }
 }
 
-__global__ __launch_bounds__(256) void k_mc_count_rows(const uint64_t* __restrict__ mask, Geom g,
-                                                       uint32_t* __restrict__ owned,
-                                                       uint64_t* __restrict__ scan_state, int64_t scan_state_words)
+/* Owned cut edges of the cell row of lane words L (row r = (j, k)), summed over the row's lanes
+ * (valid in every lane of the row): bit-sliced over the 64 cells of a word. Lanes without cells
+ * (the last row or plane, or past the row) count nothing, whatever their words. */
+__device__ __forceinline__ uint32_t mc_owned_row_count(const LaneWords& L, const Geom& g, const RowLanes& rl, bool ok,
+                                                       bool has_y, bool has_z, int64_t j, int64_t k)
 {
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < scan_state_words;
-         t += (int64_t)gridDim.x * blockDim.x)
-        scan_state[t] = 0;
-    const RowLanes rl = row_lanes(g.W);
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave_in_block()) * rl.rpw;
-    const int64_t r = r0 + rl.q;
-    const int64_t R = g.ny * g.nz;
-    const bool ok = r < R && rl.w < g.W;
-    const RowJK jk = split_row(r0, rl.q, g.ny);
-    const int64_t j = jk.j, k = jk.k;
-    const bool has_y = j < g.ny - 1, has_z = k < g.nz - 1;
-    const LaneWords L = load_lane_words(mask, g, r, rl.w, ok, has_y && has_z, has_y && has_z);
     const CornerWords C = corner_words(L);
     const uint64_t eb = (ok && has_y && has_z) ? low_bits(g.nx - 1 - 64 * rl.w) : 0ull;
     const uint64_t first_cell = rl.w == 0 ? 1ull : 0ull;
@@ -80,8 +71,33 @@ __global__ __launch_bounds__(256) void k_mc_count_rows(const uint64_t* __restric
         if (need_i) own &= first_cell;
         cnt += (uint32_t)__popcll(cut & own);
     }
-    cnt = seg_sum(cnt, rl.Wp);
-    if (r < R && rl.w == 0) owned[r] = cnt;
+    return seg_sum(cnt, rl.Wp);
+}
+
+/* Flying Edges pass 2 (k_fe_count_rows) and the first-visit owned-edge counts per cell row in
+ * one pass over the mask words, each row's words loaded once. */
+__global__ __launch_bounds__(256) void k_fe_mc_count_rows(const uint64_t* __restrict__ mask, Geom g,
+                                                          RowCounts* __restrict__ counts,
+                                                          RowTrim* __restrict__ trims,
+                                                          uint64_t* __restrict__ scan_state,
+                                                          int64_t scan_state_words, uint32_t* __restrict__ owned,
+                                                          uint64_t* __restrict__ mc_scan_state,
+                                                          int64_t mc_scan_state_words)
+{
+    // the scans of both numberings run next on the same stream; clear their tickets and status
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < scan_state_words;
+         t += (int64_t)gridDim.x * blockDim.x)
+        scan_state[t] = 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < mc_scan_state_words;
+         t += (int64_t)gridDim.x * blockDim.x)
+        mc_scan_state[t] = 0;
+    const int64_t R = g.ny * g.nz;
+    fe_count_rows_block(mask, g, counts, trims, blockIdx.x,
+                        [&](const LaneWords& L, const RowLanes& rl, int64_t r, bool ok, bool has_y, bool has_z,
+                            int64_t j, int64_t k, bool empty) {
+                            const uint32_t cnt = empty ? 0u : mc_owned_row_count(L, g, rl, ok, has_y, has_z, j, k);
+                            if (r < R && rl.w == 0) owned[r] = cnt;
+                        });
 }
 
 constexpr int kMcCellCap = 512;
