@@ -191,6 +191,27 @@ def bench_c2(reps=10):
     return out
 
 
+def stream_copy_gbs(nbytes=2 << 30, reps=10):
+    """Measured stream peak beside the published 8 TB/s: bytes read plus written per second by a
+    plain device-to-device copy of 2 GiB (torch), timed with events on the current stream."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * nbytes * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
 def pass_breakdown(algo, reps):
     """Average device time of each pass over `reps` runs with all four passes timed (HIP event
     pairs around each kernel), then timing left off; the runs are not part of any timed region."""
@@ -535,6 +556,7 @@ def main():
     achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
     kern_total = float(pass_ms.sum())
     traffic = read_traffic(dom) if (n == 1024 and args.iso == 0.5) else None
+    copy_gbs = stream_copy_gbs()
 
     # the sparse isovalue of the same config
     sparse = None
@@ -574,7 +596,9 @@ def main():
         "extraction": {"bytes_alg": b_alg, "kernel_ms": round(kern_total, 4),
                        "hbm_frac_kernels": round(b_alg / (kern_total * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4),
-                       "pass_ms": [round(float(x), 4) for x in pass_ms]},
+                       "pass_ms": [round(float(x), 4) for x in pass_ms],
+                       "stream_copy_gb_s": round(copy_gbs, 1),
+                       "frac_of_stream_copy_wall": round(b_alg / wall / 1e9 / copy_gbs, 4)},
         "sparse_iso_0.95": sparse,
         "volume_gen_s": round(gen_s, 2),
     }
