@@ -43,7 +43,7 @@ __device__ __forceinline__ RowLanes row_lanes(int64_t W)
 __device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
 /* a / b for values that are the same in every lane (scalar code; 32-bit when both fit) */
-__device__ __forceinline__ int64_t uniform_div(int64_t a, int64_t b)
+__host__ __device__ __forceinline__ int64_t uniform_div(int64_t a, int64_t b)
 {
     if (((uint64_t)a | (uint64_t)b) >> 32 == 0) return (int64_t)((uint32_t)a / (uint32_t)b);
     return a / b;
@@ -55,7 +55,7 @@ struct RowJK {
     int64_t j, k;
 };
 
-__device__ __forceinline__ RowJK split_row(int64_t r0, int q, int64_t ny)
+__host__ __device__ __forceinline__ RowJK split_row(int64_t r0, int q, int64_t ny)
 {
     RowJK p;
     p.k = uniform_div(r0, ny);

@@ -1,7 +1,9 @@
 """CPU: the order in which the lane-row pass 4 kernels' blocks take the grid rows
 (plan_block_order / emit_block_order, miniisosurface_amd/csrc/fe_rows.h) is a permutation of the
-blocks for every row width and plane size, so that every row is written exactly once. The header
-is compiled for the host with hipcc; the kernels' own parity tests cover the order on the GPU."""
+blocks for every row width and plane size, so that every row is written exactly once; and the
+row kernels' (j, k) of grid row r0 + q from one division per wavefront (split_row) equals
+r mod ny, r / ny. The header is compiled for the host with hipcc; the kernels' own parity tests
+cover both on the GPU."""
 import os
 import shutil
 import subprocess
@@ -41,6 +43,17 @@ int main()
                     }
                     seen[a] = 1;
                     tiled += a != b;
+                }
+            }
+    // split_row against division, across 32-bit and 64-bit row indices and small ny (carries)
+    for (int64_t ny : {1ll, 2ll, 3ll, 7ll, 63ll, 64ll, 65ll, 1000ll, 4096ll, 70000ll, 5000000000ll})
+        for (int64_t r0 : {0ll, 1ll, 63ll, 64ll, 4095ll, 123456ll, 4294967295ll, 4294967296ll, 9000000000123ll})
+            for (int q = 0; q < 64; ++q) {
+                const miso::RowJK p = miso::split_row(r0, q, ny);
+                const int64_t r = r0 + q;
+                if (p.j != r % ny || p.k != r / ny) {
+                    printf("split_row(%ld, %d, %ld) = (%ld, %ld)\n", r0, q, ny, p.j, p.k);
+                    return 1;
                 }
             }
     printf("tiled %ld\n", tiled);
