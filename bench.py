@@ -328,7 +328,7 @@ def bench_host_scope(vol, iso, reps=2):
             "volume_bytes": int(host.nbytes), "mesh_bytes": int(mesh.points.nbytes * 2 + mesh.triangles.nbytes)}
 
 
-def bench_batched(reps):
+def bench_batched(reps, cpu_threads=None):
     """256 mixed-size Gaussian volumes (dims log-uniform in [32, 256], 1-6 blobs, iso U[0.2, 0.8],
     seed 0), and one 512^3 gyroid at 16 isovalues in [-1, 1]: the batch (one persistent launch per
     pass, graph replay, autotuned shape) against the same jobs run one extraction at a time."""
@@ -397,6 +397,17 @@ def bench_batched(reps):
     mixed = run_set("mixed", jobs, algos)
     mixed["voxels"] = voxels
     del algos
+    if cpu_threads:
+        # the same volumes on the host: the reference's OpenMP Flying Edges, one volume after
+        # another on all threads, by its own timer, on a bounded sample (every 8th volume)
+        sample = vols[::8]
+        secs, kind = 0.0, None
+        for t, iso in sample:
+            sec, kind = _reference_fe_seconds("flyingEdgesOpenMP", t.cpu().numpy(), iso, cpu_threads, 1)
+            secs += sec
+        mixed["cpu_baseline"] = {"value": round(len(sample) / secs, 1), "unit": "volumes/s", "cores": cpu_threads,
+                                 "kind": kind, "sample": f"every 8th of the 256 volumes ({len(sample)}), one run "
+                                 f"each, OMP_PROC_BIND=close, OMP_PLACES=cores ({secs:.3f} s)"}
     g = synth.gyroid(512, seed=0, device="cuda")
     dv = m.DeviceVolume.from_tensor(g)
     isos = [float(x) for x in np.linspace(-1.0, 1.0, 16)]
@@ -611,7 +622,8 @@ def main():
     if not args.no_c2:
         out["c2_gyroid_512_iso0"] = bench_c2()
     if not args.no_batch:
-        out["batched"] = bench_batched(reps=3)
+        out["batched"] = bench_batched(
+            reps=3, cpu_threads=None if args.no_cpu else int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     if not args.no_serve:
         out["serving"] = bench_serving(args.serve_requests)
     if not args.no_cpu:
