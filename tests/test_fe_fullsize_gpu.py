@@ -110,3 +110,39 @@ def test_fe_1024_sparse_bit_exact_on_active_box(oracle_mod):
     assert bits_equal(mesh.points, ref.points)
     assert bits_equal(mesh.normals, ref.normals)
     assert np.array_equal(mesh.triangles, ref.triangles)
+
+
+def test_fe_1024_dense_slab_bit_exact(oracle_mod):
+    """The benchmark volume at iso 0.5, bit for bit against the oracle on a slab: the vertices of
+    the grid rows of planes k0..k1 and the triangles of the cell rows of planes k0..k1 equal the
+    oracle's for the sub-volume of planes k0 - 2 .. k1 + 3 (origin z = k0 - 2). Every gradient
+    those vertices use lies inside the sub-volume, away from its z faces (the x and y faces are the
+    volume's own), and rows keep their order, so points and normals match bitwise and triangle
+    ids match relative to each range's first vertex."""
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+    from helpers import bits_equal
+
+    n, iso = 1024, 0.5
+    k0, k1 = 509, 512
+    vol = synth.value_noise(n, seed=0, device="cuda")
+    algo = m.FlyingEdgesAlgorithm(m.DeviceVolume.from_tensor(vol), iso)
+    algo.run()
+    mesh = algo.moveOutput()
+    rows = algo.rows()
+    ny = n
+    gv0, gv1 = int(rows["voffset"][k0 * ny]), int(rows["voffset"][(k1 + 1) * ny])
+    gt0, gt1 = int(rows["toffset"][k0 * ny]), int(rows["toffset"][(k1 + 1) * ny])
+    kb, ke = k0 - 2, k1 + 4
+    sub = vol[kb:ke].contiguous().cpu().numpy()
+    del vol
+    ref = oracle_mod.FEResult(sub, iso, (1.0, 1.0, 1.0), (0.0, 0.0, float(kb)), nthreads=4, want_debug=True)
+    sv0, sv1 = int(ref.rows["xstart"][(k0 - kb) * ny]), int(ref.rows["xstart"][(k1 + 1 - kb) * ny])
+    st0, st1 = int(ref.rows["toffset"][(k0 - kb) * (ny - 1)]), int(ref.rows["toffset"][(k1 + 1 - kb) * (ny - 1)])
+    print(f"[dense slab] planes {k0}..{k1}: {gv1 - gv0} vertices, {gt1 - gt0} triangles", flush=True)
+    assert gv1 - gv0 == sv1 - sv0 > 0 and gt1 - gt0 == st1 - st0 > 0
+    assert bits_equal(mesh.points[gv0:gv1], ref.points[sv0:sv1])
+    assert bits_equal(mesh.normals[gv0:gv1], ref.normals[sv0:sv1])
+    gtri = mesh.triangles[gt0:gt1].astype(np.int64) - gv0
+    stri = ref.triangles[st0:st1].astype(np.int64) - sv0
+    assert np.array_equal(gtri, stri)
