@@ -3,11 +3,15 @@
 Contract: `python bench.py [--gpus 1] [--steps K] [--warmup W]` prints ONE JSON line.
 A step is one complete extraction (passes 1-4: classify, count, scan, emit, with the host
 synchronisation after the scan that sizes the output) of the volume, already resident in HBM.
-value = cells / (wall time per step), in Gcells/s. The roofline object reports the dominant
-kernel's algorithmic bytes over its average duration, measured with HIP events recorded by the
-library on the stream the kernels run on. In the timed steps only that kernel (pass 4) is timed
-with events; the per-pass breakdown comes from as many steps, run before them, with all four
-passes timed (each event pair adds a few microseconds between the kernels).
+value = cells / (wall time per step), in Gcells/s. The roofline object reports the extraction's
+compulsory HBM bytes, B_alg = 4N + 24V + 12T (the volume read once, points and normals, 32-bit
+triangle indices), over the summed device time of its four kernels, measured with HIP event
+pairs that the library records around each kernel on the stream the kernels run on (steps run
+before the timed ones; each event pair adds a few microseconds between the kernels, so the
+timed steps carry none but pass 4's). `roofline.kernels` gives each kernel its own compulsory
+bytes (pass 1: the volume and the sign mask; pass 2: mask and row counts; pass 3: row state;
+pass 4: points, normals, triangles, the mask and the row state it reads), its time and rate,
+and the HBM bytes its rocprofv3 counters show per launch.
 """
 from __future__ import annotations
 
@@ -53,13 +57,59 @@ def _reference_fe_seconds(exe_name, vol_np, iso, threads, runs):
     return sorted(times)[len(times) // 2], kind
 
 
+def host_cores():
+    """(threads used, physical cores those threads may run on, logical CPUs of the host): the
+    threads are OMP_NUM_THREADS (the host CPU share given to one GPU), else the CPUs this
+    process may run on; physical cores are counted from `lscpu -p=CPU,CORE,SOCKET` over that
+    CPU set."""
+    allowed = sorted(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(allowed)))
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        core_of = {}
+        for line in out.splitlines():
+            if line and not line.startswith("#"):
+                c, core, sock = line.split(",")[:3]
+                core_of[int(c)] = (core, sock)
+        phys = len({core_of[c] for c in allowed if c in core_of})
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pass
+    return threads, phys, os.cpu_count()
+
+
+def _reference_mc_seconds(exe_name, vol_np, iso, threads, runs):
+    """Median wall time of a reference Marching Cubes driver (oracle/_ref) by its own timer
+    (MarchingCubes call only, file I/O excluded), or None when it is not built."""
+    from oracle import ora
+    from miniisosurface_amd import vtkio
+    exe = ora.ref_binary(exe_name)
+    if not exe:
+        return None
+    times = []
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
+        vtkio.save_image(os.path.join(tmp, "v.vtk"), vtkio.Image3D(vol_np))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        for _ in range(runs):
+            out = subprocess.run([exe, "-i", "v.vtk", "-o", "m.vtk", "-v", repr(float(iso)),
+                                  "-y", os.path.join(tmp, "y", "r.yaml")], cwd=tmp, env=env,
+                                 check=True, capture_output=True, text=True).stdout
+            for line in out.splitlines():
+                if "Total Program WALL Time (seconds)" in line:
+                    times.append(float(line.split(":")[-1]))
+    return sorted(times)[len(times) // 2]
+
+
 def cpu_baseline(vol_np, iso, threads, runs=5):
     """The reference's OpenMP Flying Edges on a bounded slab of the same volume, all threads."""
     nz, ny, nx = vol_np.shape
     cells = (nx - 1) * (ny - 1) * (nz - 1)
     t, kind = _reference_fe_seconds("flyingEdgesOpenMP", vol_np, iso, threads, runs)
+    _, phys, logical = host_cores()
     return {"value": round(cells / t / 1e9, 4), "unit": "Gcells/s", "cores": threads, "kind": kind,
-            "sample": f"{nx}x{ny}x{nz} slab of the same volume, iso {iso}, median of {runs} runs, "
+            "physical_cores_available": phys, "host_logical_cpus": logical,
+            "sample": f"measured on a {nx}x{ny}x{nz} slab of the same volume (a per-cell rate, not a "
+                      f"whole-volume run), iso {iso}, median of {runs} runs, {threads} OpenMP threads, "
                       f"OMP_PROC_BIND=close, OMP_PLACES=cores, timer = reference ctor..moveOutput ({t:.3f} s)"}
 
 
@@ -76,6 +126,8 @@ def bench_c1(threads, runs=5, reps=20):
     cells = 127 ** 3
     ts, kind = _reference_fe_seconds("flyingEdgesSerial", vol, 0.5, 1, runs)
     to, _ = _reference_fe_seconds("flyingEdgesOpenMP", vol, 0.5, threads, runs)
+    mcs = _reference_mc_seconds("mcSerial", vol, 0.5, 1, runs)
+    mco = _reference_mc_seconds("mcOpenMP", vol, 0.5, threads, runs)
     algo = m.FlyingEdgesAlgorithm(m.Image3D(vol), 0.5)
     algo.set_timing(())   # no event pairs between the passes in the timed runs
     for _ in range(3):
@@ -93,6 +145,8 @@ def bench_c1(threads, runs=5, reps=20):
     whole = (time.perf_counter() - t0) / 5
     return {"nverts": mesh.numberOfVertices(), "ntris": mesh.numberOfTriangles(), "kind": kind,
             "reference_serial_ms": round(ts * 1e3, 3), "reference_openmp_ms": round(to * 1e3, 3),
+            "reference_mc_serial_ms": None if mcs is None else round(mcs * 1e3, 3),
+            "reference_mc_openmp_ms": None if mco is None else round(mco * 1e3, 3),
             "openmp_threads": threads, "gpu_ms": round(gpu * 1e3, 4),
             "gpu_with_copies_ms": round(whole * 1e3, 3),
             "gpu_gcells_s": round(cells / gpu / 1e9, 2)}
@@ -156,7 +210,7 @@ def read_traffic(kernel):
     return None
 
 
-def bench_c2(reps=10):
+def bench_c2(reps=10, cpu_threads=None):
     """512^3 gyroid (8 periods) + U(-0.05, 0.05) noise (seed 0), iso 0: Flying Edges and Marching
     Cubes (first-visit order, and edge order as the duplicate-free variant numbers vertices),
     volume resident in HBM, one complete extraction per step."""
@@ -188,6 +242,18 @@ def bench_c2(reps=10):
                      "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4), "nverts": V, "ntris": T}
         if name == "flying_edges":
             out[name]["pass_ms"] = pass_breakdown(algo, reps)
+    if cpu_threads:
+        # the reference's OpenMP drivers on the same volume, on the host (one full run each, the
+        # median of 3; Marching Cubes with its default sections: one z-section per thread)
+        host = g.cpu().numpy()
+        tf, kind = _reference_fe_seconds("flyingEdgesOpenMP", host, 0.0, cpu_threads, 3)
+        tm = _reference_mc_seconds("mcOpenMP", host, 0.0, cpu_threads, 3)
+        out["cpu_baseline"] = {"unit": "Gcells/s", "cores": cpu_threads, "kind": kind,
+                               "flying_edges_openmp": round(cells / tf / 1e9, 4),
+                               "marching_cubes_openmp": None if tm is None else round(cells / tm / 1e9, 4),
+                               "sample": f"the whole 512^3 volume, median of 3 runs each, {cpu_threads} OpenMP "
+                                         f"threads, timer = the drivers' own ({tf:.3f} s FE, "
+                                         f"{'-' if tm is None else f'{tm:.3f}'} s MC)"}
     return out
 
 
@@ -419,47 +485,52 @@ def bench_batched(reps, cpu_threads=None):
 def bench_serving(n_requests, rate=200.0, seed=0, saturated_rate=20000.0):
     """Poisson arrivals (rate 200/s, seeded) over a resident pool of 16 volumes with cubic dims
     drawn from {64, 128, 256, 384, 512} (Gaussian blobs or gyroid), isovalue uniform over each
-    field's [min, max], 200 GB admission budget."""
+    field's [min, max], 200 GB admission budget. Latency runs from arrival to delivery: with
+    host delivery (the headline figures) until the request's mesh has landed in pinned host
+    memory; with device hand-over (`device_handoff`) until pass 4 has written it, the requester
+    then holding its slice of the device output (released at once here)."""
     import numpy as np
     import torch
 
     from miniisosurface_amd import DeviceVolume, synth
     from miniisosurface_amd.serve import IsosurfaceServer, latency_stats, poisson_requests
 
-    rng = np.random.default_rng(seed)
-    pool, ranges, dims = [], [], []
-    for q in range(16):
-        n = int(rng.choice([64, 128, 256, 384, 512]))
-        if rng.uniform() < 0.5:
-            t = synth.gaussians(n, count=int(rng.integers(1, 7)), seed=int(rng.integers(1 << 30)),
-                                sigma_range=(8.0 * n / 128, 20.0 * n / 128), device="cuda")
-        else:
-            t = synth.gyroid(n, seed=int(rng.integers(1 << 30)), device="cuda")
-        pool.append(DeviceVolume.from_tensor(t))
-        ranges.append((float(t.min()), float(t.max())))
-        dims.append(n)
+    pool = synth.serving_pool(16, seed=seed, device="cuda")
+    dvs = [DeviceVolume.from_tensor(t) for t, _, _ in pool]
+    ranges = [r for _, r, _ in pool]
+    dims = [n for _, _, n in pool]
     torch.cuda.synchronize()
-    reqs = poisson_requests(n_requests, rate, ranges, seed=seed + 1)
-    srv = IsosurfaceServer(pool, budget_bytes=200e9, max_jobs=64)
-    srv.run(poisson_requests(8, 1e4, ranges, seed=seed + 2))     # warm-up: slot buffers, kernels
-    srv.steps, srv.step_sizes = 0, []
-    done = srv.run(reqs)
-    st = latency_stats(done)
-    st.update({"rate_per_s": rate, "pool_dims": dims, "steps": srv.steps,
-               "mean_batch": round(float(np.mean(srv.step_sizes)), 2) if srv.step_sizes else 0,
-               "max_batch": int(max(srv.step_sizes)) if srv.step_sizes else 0,
-               "peak_live_gb": round(srv.peak_live_bytes / 1e9, 3), "budget_gb": 200})
+
+    def play(deliver, reqs, on_done=None):
+        srv = IsosurfaceServer(dvs, budget_bytes=200e9, max_jobs=64, deliver=deliver)
+        srv.run(poisson_requests(8, 1e4, ranges, seed=seed + 2), on_done=on_done)   # warm-up
+        srv.steps, srv.step_sizes, srv.peak_live_bytes = 0, [], 0
+        done = srv.run(reqs, on_done=on_done)
+        st = latency_stats(done)
+        st.update({"rate_per_s": round(len(reqs) / max(reqs[-1].arrival, 1e-9), 1), "steps": srv.steps,
+                   "mean_batch": round(float(np.mean(srv.step_sizes)), 2) if srv.step_sizes else 0,
+                   "max_batch": int(max(srv.step_sizes)) if srv.step_sizes else 0,
+                   "peak_live_gb": round(srv.peak_live_bytes / 1e9, 3), "deliver": deliver})
+        delivered = sum(r.nverts * 24 + r.ntris * 12 for r in done if not r.rejected)
+        st["delivered_gb"] = round(delivered / 1e9, 3)
+        for r in done:
+            r.result = None
+        srv.close()
+        return st
+
+    def consume(r):
+        # the requester takes its mesh and lets go of it (the pinned blocks are then reused)
+        r.result = None
+
+    st = play("host", poisson_requests(n_requests, rate, ranges, seed=seed + 1), on_done=consume)
+    st.update({"pool_dims": dims, "budget_gb": 200, "offered_rate_per_s": rate})
+    st["device_handoff"] = play("device", poisson_requests(n_requests, rate, ranges, seed=seed + 1),
+                                on_done=lambda r: r.release())
     # the same pool and request mix offered faster than one-at-a-time service keeps up with:
     # arrivals queue while the device works and are served in batches
-    srv.steps, srv.step_sizes, srv.peak_live_bytes = 0, [], 0
-    done = srv.run(poisson_requests(2 * n_requests, saturated_rate, ranges, seed=seed + 3))
-    sat = latency_stats(done)
-    sat.update({"rate_per_s": saturated_rate, "steps": srv.steps,
-                "mean_batch": round(float(np.mean(srv.step_sizes)), 2) if srv.step_sizes else 0,
-                "max_batch": int(max(srv.step_sizes)) if srv.step_sizes else 0,
-                "peak_live_gb": round(srv.peak_live_bytes / 1e9, 3)})
-    st["saturated"] = sat
-    srv.close()
+    st["saturated"] = play("host", poisson_requests(2 * n_requests, saturated_rate, ranges, seed=seed + 3),
+                           on_done=consume)
+    st["saturated"]["offered_rate_per_s"] = saturated_rate
     return st
 
 
@@ -546,27 +617,37 @@ def main():
     V, T = algo.counts()
     _, _, _, bits, _, _ = algo.device_output()
 
-    # algorithmic bytes (compulsory traffic) of one extraction and of each pass
-    b_alg = 4 * N + 24 * V + (bits // 8) * 3 * T
+    # compulsory HBM bytes of one extraction and of each pass (uint32 or uint64 triangle ids)
+    tri_b = (bits // 8) * 3 * T
+    b_alg = 4 * N + 24 * V + tri_b
     R = n * n
     W = (n + 63) // 64
     mask_b = 8 * W * R
-    rows = algo.rows()
-    # grid rows holding a vertex: pass 4 must read those rows of the volume (the gradient stencil
-    # reaches their neighbours, which are almost always active rows themselves at this density)
-    active_rows = int(np.count_nonzero(rows["xcount"].astype(np.int64) + rows["ycount"] + rows["zcount"]))
     per_kernel = {
+        # the volume once, the sign mask written
         "miso::k_fe_classify": {"ms": pass_ms[0], "bytes": 4 * N + mask_b},
+        # the mask read, counts and trims written (16 + 8 bytes per grid row)
         "miso::k_fe_count_rows": {"ms": pass_ms[1], "bytes": mask_b + 24 * R},
+        # counts read, vertex and triangle offsets written
         "miso::k_fe_scan": {"ms": pass_ms[2], "bytes": 16 * R + 16 * R},
-        "miso::k_fe_emit_rows": {"ms": timed_ms[3],
-                                 "bytes": 4 * n * active_rows + 24 * V + (bits // 8) * 3 * T + mask_b},
+        # points, normals, triangles written; the mask, counts and offsets read. Its reads of
+        # the volume near the surface are not compulsory: pass 1 has read it once already
+        "miso::k_fe_emit_rows": {"ms": timed_ms[3], "bytes": 24 * V + tri_b + mask_b + 32 * R},
     }
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"])
-    dk = per_kernel[dom]
-    achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
     kern_total = float(pass_ms.sum())
-    traffic = read_traffic(dom) if (n == 1024 and args.iso == 0.5) else None
+    traffic_total = 0.0
+    kernels = []
+    for name, d in per_kernel.items():
+        tr = read_traffic(name) if (n == 1024 and args.iso == 0.5) else None
+        if tr is None:
+            traffic_total = None
+        elif traffic_total is not None:
+            traffic_total += tr
+        kernels.append({"kernel": name, "ms": round(float(d["ms"]), 4), "alg_bytes": int(d["bytes"]),
+                        "achieved_gb_s": round(d["bytes"] / (d["ms"] * 1e-3) / 1e9, 1),
+                        "frac": round(d["bytes"] / (d["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "traffic_bytes": None if tr is None else int(tr)})
+    achieved = b_alg / (kern_total * 1e-3) / 1e9
     copy_gbs = stream_copy_gbs()
 
     # the sparse isovalue of the same config
@@ -600,14 +681,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"flying edges, {n}^3 float32 3-octave value noise (seed 0), iso {args.iso}",
                    "nverts": V, "ntris": T, "index_bits": bits},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None if traffic is None else int(traffic),
-                     "alg_bytes": int(dk["bytes"]), "kernel_ms": round(float(dk["ms"]), 4)},
+        "roofline": {"bound": "hbm", "kernel": "extraction: " + " + ".join(k["kernel"] for k in kernels),
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic_total is None else int(traffic_total),
+                     "alg_bytes": int(b_alg), "kernel_ms": round(kern_total, 4),
+                     "traffic_over_alg": None if traffic_total is None else round(traffic_total / b_alg, 3),
+                     "kernels": kernels},
         "extraction": {"bytes_alg": b_alg, "kernel_ms": round(kern_total, 4),
                        "hbm_frac_kernels": round(b_alg / (kern_total * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4),
                        "pass_ms": [round(float(x), 4) for x in pass_ms],
+                       "pass4_ms_timed_steps": round(float(timed_ms[3]), 4),
                        "stream_copy_gb_s": round(copy_gbs, 1),
                        "frac_of_stream_copy_wall": round(b_alg / wall / 1e9 / copy_gbs, 4)},
         "sparse_iso_0.95": sparse,
@@ -619,16 +704,15 @@ def main():
         out["slabs_128_planes"] = bench_slabs(vol, args.iso)
     if not args.no_large:
         out["large_2048_gyroid_iso0.1"] = bench_large()
+    threads = host_cores()[0]
     if not args.no_c2:
-        out["c2_gyroid_512_iso0"] = bench_c2()
+        out["c2_gyroid_512_iso0"] = bench_c2(cpu_threads=None if args.no_cpu else threads)
     if not args.no_batch:
-        out["batched"] = bench_batched(
-            reps=3, cpu_threads=None if args.no_cpu else int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        out["batched"] = bench_batched(reps=3, cpu_threads=None if args.no_cpu else threads)
     if not args.no_serve:
         out["serving"] = bench_serving(args.serve_requests)
     if not args.no_cpu:
         slab = vol[: max(2, n // 4)].cpu().numpy()
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(slab, args.iso, threads)
         out["cpu_baseline_1_thread"] = cpu_baseline(slab[: max(2, n // 16)], args.iso, 1)
         out["c1_gaussians_128_iso0.5"] = bench_c1(threads)

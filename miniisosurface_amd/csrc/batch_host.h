@@ -115,6 +115,9 @@ int enqueue_emit_into(miso_batch* b, int64_t nemit, float* pts, float* nrm, uint
     const int64_t nlane = b->lane_prefix[nemit];
     if (nlane == 0) return MISO_OK;
     const uint64_t* need = nemit < b->njobs ? b->d_jobout + 4 * nemit : b->totals();
+    // the emit queue's ticket starts at zero on every emit, also a second emit after one count
+    // (a fill kernel, not a memset node, so that the enqueue can be captured)
+    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, b->stream, b->tickets() + 2, (int64_t)1);
     hipLaunchKernelGGL(k_batch_emit, dim3(batch_grid(b, nlane)), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
                        (const int64_t*)b->d_lane_prefix, b->njobs, nlane, b->chunk, (const uint64_t*)b->d_mask,
                        (const RowCounts*)b->d_counts, (const uint64_t*)b->d_voff, (const uint64_t*)b->d_toff,

@@ -1,6 +1,6 @@
 """Serving loop: continuous batching of isosurface requests over volumes resident in HBM.
 
-    server = IsosurfaceServer(pool, budget_bytes=200e9)
+    server = IsosurfaceServer(pool, budget_bytes=200e9, deliver="host")
     done = server.run(requests)          # requests: Request(volume index, isovalue, arrival s)
     stats = latency_stats(done)
 
@@ -11,6 +11,15 @@ while the memory their output needs (24 bytes per vertex, 12 per triangle) fits 
 by the outputs still alive, pass 4 of that prefix is enqueued into one freshly allocated output
 arena, and the rest go back to the head of the queue. With two slots, the counts of newly
 arrived requests run on the device while the previous step's triangles are being written.
+
+A request is done when its mesh is delivered, and its latency runs from arrival to delivery:
+  deliver="host"    the step's output is copied into pinned host memory on the slot's stream
+                    right behind pass 4; the request is done when that copy has landed, and
+                    holds its mesh as host arrays (Request.result, Request.mesh); the device
+                    arena is released then;
+  deliver="device"  the request is done when pass 4 has finished, and holds device views of its
+                    slice of the arena (Request.result); the arena's bytes stay counted against
+                    the budget until every request of the step has called Request.release().
 
 The reference has no serving path; each request's result is bit-identical to
 FlyingEdgesAlgorithm on its volume and isovalue (flyingEdges/serial/FlyingEdgesAlgorithm.h:27-49).
@@ -43,29 +52,64 @@ class Request:
     ntris: int = 0
     rejected: bool = False
     step: int = -1
-    mesh: TriangleMesh | None = None
+    result: tuple | None = None      # (points (V, 3), normals (V, 3), triangles (T, 3) uint32)
+    _arena: object = field(default=None, repr=False)
 
     @property
     def latency(self):
         return None if self.done is None else self.done - self.arrival
 
+    @property
+    def mesh(self) -> TriangleMesh | None:
+        """The delivered mesh with the reference's 64-bit triangle indices (host delivery)."""
+        if self.result is None:
+            return None
+        p, n, t = self.result
+        if not isinstance(p, np.ndarray):
+            p, n, t = p.cpu().numpy(), n.cpu().numpy(), t.cpu().numpy()
+        return TriangleMesh(p, n, t.view(np.uint32).astype(np.uint64))
+
+    def release(self):
+        """Gives back a device-delivered result: the step's arena is freed once all of its
+        requests have released theirs."""
+        a, self._arena = self._arena, None
+        self.result = None
+        if a is not None:
+            a.release()
+
 
 class _Arena:
     """Device output of one step: points, normals, triangles of its admitted jobs."""
 
-    def __init__(self, nv, nt, device):
+    def __init__(self, nv, nt, device, server):
         import torch
         self.pts = torch.empty(max(3 * nv, 1), dtype=torch.float32, device=device)
         self.nrm = torch.empty(max(3 * nv, 1), dtype=torch.float32, device=device)
         self.tris = torch.empty(max(3 * nt, 1), dtype=torch.int32, device=device)
         self.bytes = nv * BYTES_PER_VERTEX + nt * BYTES_PER_TRIANGLE
         self.nv, self.nt = nv, nt
+        self.server = server
+        self.holders = 0
+        self.host = None
+
+    def release(self):
+        """One holder less; the last one returns the arena's bytes to the budget."""
+        self.holders -= 1
+        if self.holders <= 0 and self.server is not None:
+            self.server.live_bytes -= self.bytes
+            self.server._released = True
+            self.pts = self.nrm = self.tris = None
+            self.server = None
 
 
 class _Slot:
     def __init__(self):
+        import torch
+        # a stream of the default priority, shared by the batch's kernels and the delivery copies
+        self.stream = torch.cuda.Stream()
         h = ctypes.c_void_p()
-        check(lib().miso_batch_create_empty(None, ctypes.byref(h)), "serving slot")
+        check(lib().miso_batch_create_empty(ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)),
+              "serving slot")
         self.h = h
         self.state = "idle"
         self.reqs = []
@@ -86,9 +130,12 @@ class _Slot:
 
 
 class IsosurfaceServer:
-    def __init__(self, pool, budget_bytes=200e9, max_jobs=64, slots=2, keep_meshes=False, device="cuda"):
-        """pool: list of DeviceVolume. keep_meshes copies every result to the host (tests)."""
+    def __init__(self, pool, budget_bytes=200e9, max_jobs=64, slots=2, deliver="host", device="cuda"):
+        """pool: list of DeviceVolume. deliver: "host" or "device" (see the module text)."""
         from . import DeviceVolume
+        if deliver not in ("host", "device"):
+            raise ValueError('deliver is "host" or "device"')
+        self.deliver = deliver
         self.pool = list(pool)
         for v in self.pool:
             if not isinstance(v, DeviceVolume):
@@ -98,8 +145,8 @@ class IsosurfaceServer:
         torch.cuda.synchronize()
         self.budget = int(budget_bytes)
         self.max_jobs = int(max_jobs)
-        self.keep_meshes = keep_meshes
         self.device = device
+        self._released = False
         self.slots = [_Slot() for _ in range(slots)]
         self.live_bytes = 0
         self.peak_live_bytes = 0
@@ -156,11 +203,19 @@ class IsosurfaceServer:
             return [r]
         waiting.extendleft(reversed(slot.reqs[k:]))
         slot.reqs = slot.reqs[:k]
-        arena = _Arena(nv, nt, self.device)
+        arena = _Arena(nv, nt, self.device, self)
         self.live_bytes += arena.bytes
         self.peak_live_bytes = max(self.peak_live_bytes, self.live_bytes)
         check(lib().miso_batch_emit_async(slot.h, k, arena.pts.data_ptr(), arena.nrm.data_ptr(),
                                           arena.tris.data_ptr(), max(nv, 1), max(nt, 1)), "emit")
+        if self.deliver == "host":
+            # delivery: the step's output to pinned host memory, on the slot's stream behind pass 4
+            import torch
+            with torch.cuda.stream(slot.stream):
+                arena.host = tuple(torch.empty(t.numel(), dtype=t.dtype, pin_memory=True)
+                                   for t in (arena.pts, arena.nrm, arena.tris))
+                for h, d in zip(arena.host, (arena.pts, arena.nrm, arena.tris)):
+                    h.copy_(d, non_blocking=True)
         slot.arena = arena
         slot.step = self.steps
         self.steps += 1
@@ -173,27 +228,41 @@ class IsosurfaceServer:
         return []
 
     def _complete(self, slot, now):
+        """The step's pass 4 (and, for host delivery, its copy to the host) has finished: each
+        request receives its slice of the output."""
         check(lib().miso_batch_wait(slot.h), "wait")
         a = slot.arena
         fv, ft = ctypes.c_uint64(), ctypes.c_uint64()
+        src = a.host if self.deliver == "host" else (a.pts, a.nrm, a.tris)
         for q, r in enumerate(slot.reqs):
+            check(lib().miso_batch_job_offsets(slot.h, q, ctypes.byref(fv), ctypes.byref(ft)), "offsets")
+            v0, t0 = fv.value, ft.value
+            pts = src[0][3 * v0:3 * (v0 + r.nverts)].view(-1, 3)
+            nrm = src[1][3 * v0:3 * (v0 + r.nverts)].view(-1, 3)
+            tri = src[2][3 * t0:3 * (t0 + r.ntris)].view(-1, 3)
+            if self.deliver == "host":
+                r.result = (pts.numpy(), nrm.numpy(), tri.numpy().view(np.uint32))
+            else:
+                r.result = (pts, nrm, tri)
+                r._arena = a
+                a.holders += 1
             r.done = now
-            if self.keep_meshes:
-                check(lib().miso_batch_job_offsets(slot.h, q, ctypes.byref(fv), ctypes.byref(ft)), "offsets")
-                v0, t0 = fv.value, ft.value
-                pts = a.pts[3 * v0:3 * (v0 + r.nverts)].view(-1, 3).cpu().numpy()
-                nrm = a.nrm[3 * v0:3 * (v0 + r.nverts)].view(-1, 3).cpu().numpy()
-                tri = a.tris[3 * t0:3 * (t0 + r.ntris)].view(-1, 3).cpu().numpy().view(np.uint32).astype(np.uint64)
-                r.mesh = TriangleMesh(pts, nrm, tri)
         done = slot.reqs
-        # delivered: the step's output arena is released
-        self.live_bytes -= a.bytes
+        if self.deliver == "host":
+            # delivered to the host: the device arena is no longer needed
+            a.holders = 1
+            a.release()
+        elif a.holders == 0:
+            a.holders = 1
+            a.release()
         slot.arena, slot.reqs, slot.state = None, [], "idle"
         return done
 
     # ---------------------------------------------------------------- event loop
-    def run(self, requests, clock=time.perf_counter, timeout_s=600.0):
-        """Plays the requests at their arrival times; returns them completed, in completion order."""
+    def run(self, requests, clock=time.perf_counter, timeout_s=600.0, on_done=None):
+        """Plays the requests at their arrival times; returns them completed, in completion order.
+        on_done(request) is called as each request is delivered (a device-delivered request that
+        it releases frees its share of the budget at once)."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         finished = []
@@ -205,11 +274,16 @@ class IsosurfaceServer:
                 raise MisoError(f"serving loop exceeded {timeout_s} s with {total - len(finished)} requests left")
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            progressed = released = False
+            progressed = False
             for slot in self.slots:
                 if slot.state == "emitting" and slot.query():
-                    finished.extend(self._complete(slot, clock() - t0))
-                    progressed = released = True
+                    got = self._complete(slot, clock() - t0)
+                    finished.extend(got)
+                    if on_done is not None:
+                        for r in got:
+                            on_done(r)
+                    progressed = True
+            released, self._released = self._released, False
             for slot in self.slots:
                 if slot.state == "counting" and slot.query():
                     self._counted(slot)

@@ -18,6 +18,9 @@ same seed on the same host produce the same bits:
                               the sum is divided by 1.75.
   batch_volumes(count, seed)  dims log-uniform in [32, 256] per axis, 1-6 Gaussians each with
                               sigma scaled to the volume, isovalue U[0.2, 0.8].
+  serving_pool(count, seed)   the serving workload's resident volumes: cubic dims drawn from
+                              {64, 128, 256, 384, 512}, each a Gaussian-blob field (1-6 blobs,
+                              sigma scaled to the volume) or a gyroid, with equal odds.
 """
 from __future__ import annotations
 
@@ -160,4 +163,24 @@ def batch_volumes(count=256, seed=0, lo=32, hi=256, device=None):
         vol = gaussians(tuple(int(d) for d in dims), count=nb, seed=sub_seed,
                         sigma_range=(8.0 * scale, 20.0 * scale), device=device)
         out.append((vol, iso))
+    return out
+
+
+SERVING_DIMS = (64, 128, 256, 384, 512)
+
+
+def serving_pool(count=16, seed=0, device=None):
+    """Resident volumes of the serving workload: list of (volume, (min, max), n). A volume is an
+    n^3 Gaussian-blob field (1-6 blobs, sigma U[8, 20] * n / 128) or an n^3 gyroid with noise,
+    each with probability 1/2; n is drawn from SERVING_DIMS (numpy default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(SERVING_DIMS))
+        if rng.uniform() < 0.5:
+            v = gaussians(n, count=int(rng.integers(1, 7)), seed=int(rng.integers(1 << 30)),
+                          sigma_range=(8.0 * n / 128, 20.0 * n / 128), device=device)
+        else:
+            v = gyroid(n, seed=int(rng.integers(1 << 30)), device=device)
+        out.append((v, (float(v.min()), float(v.max())), n))
     return out

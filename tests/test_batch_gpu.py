@@ -152,3 +152,114 @@ def test_batch_rejects_host_and_flat_volumes():
     t = torch.zeros((1, 4, 4), dtype=torch.float32, device="cuda")
     with pytest.raises(m.MisoError):
         BatchExtraction([(DeviceVolume.from_tensor(t), 0.5)])
+
+
+def test_split_phase_emit_twice_after_one_count(oracle_mod):
+    """Split phase: one count, then pass 4 of the first k jobs, then pass 4 of every job into
+    other buffers (a step that admits more once the budget frees). Each emit starts its work
+    queue afresh, so both outputs hold whole meshes equal to the oracle's."""
+    import ctypes
+
+    import torch
+
+    from miniisosurface_amd import _volume_struct
+    from miniisosurface_amd._lib import Volume, check, lib
+    jobs, tens = _jobs(seed=21)
+    jobs = [j for j in jobs if j[3] == (1, 1, 1)][:8]
+    L = lib()
+    h = ctypes.c_void_p()
+    check(L.miso_batch_create_empty(None, ctypes.byref(h)), "create_empty")
+    arr = (Volume * len(jobs))()
+    for q, j in enumerate(jobs):
+        arr[q] = _volume_struct(j[0])[0]
+    isos = (ctypes.c_float * len(jobs))(*[j[1] for j in jobs])
+    check(L.miso_batch_set_jobs(h, arr, isos, len(jobs)), "set_jobs")
+    check(L.miso_batch_count_async(h), "count")
+    check(L.miso_batch_count_wait(h), "count_wait")
+    refs = [oracle_mod.FEResult(v, iso, sp, org, want_debug=False) for _, iso, v, sp, org in jobs]
+    v, t = ctypes.c_uint64(), ctypes.c_uint64()
+    for q, r in enumerate(refs):
+        check(L.miso_batch_job_counts(h, q, ctypes.byref(v), ctypes.byref(t)), "counts")
+        assert (v.value, t.value) == (r.nverts, r.ntris), q
+    nv_all = sum(r.nverts for r in refs)
+    nt_all = sum(r.ntris for r in refs)
+
+    def emit(k):
+        pts = torch.full((3 * max(nv_all, 1),), float("nan"), device="cuda")
+        nrm = torch.full_like(pts, float("nan"))
+        tri = torch.full((3 * max(nt_all, 1),), -1, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        check(L.miso_batch_emit_async(h, k, pts.data_ptr(), nrm.data_ptr(), tri.data_ptr(), max(nv_all, 1),
+                                      max(nt_all, 1)), "emit")
+        check(L.miso_batch_wait(h), "wait")
+        return pts, nrm, tri
+
+    fv, ft = ctypes.c_uint64(), ctypes.c_uint64()
+    for k in (3, len(jobs), len(jobs)):
+        pts, nrm, tri = emit(k)
+        for q in range(k):
+            check(L.miso_batch_job_offsets(h, q, ctypes.byref(fv), ctypes.byref(ft)), "offsets")
+            v0, t0, r = fv.value, ft.value, refs[q]
+            p = pts[3 * v0:3 * (v0 + r.nverts)].view(-1, 3).cpu().numpy()
+            n = nrm[3 * v0:3 * (v0 + r.nverts)].view(-1, 3).cpu().numpy()
+            tt = tri[3 * t0:3 * (t0 + r.ntris)].view(-1, 3).cpu().numpy().view(np.uint32).astype(np.uint64)
+            assert bits_equal(p, r.points) and bits_equal(n, r.normals) and np.array_equal(tt, r.triangles), (k, q)
+    L.miso_batch_destroy(h)
+
+
+def _threads():
+    import os
+    return min(16, os.cpu_count() or 1)
+
+
+@pytest.mark.slow
+def test_batch_mixed_256_volumes_against_oracle(oracle_mod):
+    """The batched workload: 256 Gaussian-blob volumes with dims log-uniform in [32, 256] per
+    axis and their own isovalues U[0.2, 0.8] (seed 0), as one batch with the autotuned launch
+    shape; every job's mesh bit-exact against the oracle on the same volume and isovalue."""
+    import torch
+
+    from miniisosurface_amd import DeviceVolume, synth
+    from miniisosurface_amd.batch import BatchExtraction
+    vols = synth.batch_volumes(256, seed=0, device="cuda")
+    b = BatchExtraction([(DeviceVolume.from_tensor(t), iso) for t, iso in vols])
+    b.autotune(reps=1)
+    b.run()
+    torch.cuda.synchronize()
+    print(f"[batch 256] tuning {b.tuning()}, totals {b.totals()}", flush=True)
+    for q, (t, iso) in enumerate(vols):
+        r = oracle_mod.FEResult(t.cpu().numpy(), iso, nthreads=_threads(), want_debug=False)
+        got = b.mesh(q)
+        assert got.points.shape[0] == r.nverts and got.triangles.shape[0] == r.ntris, q
+        assert bits_equal(got.points, r.points), q
+        assert bits_equal(got.normals, r.normals), q
+        assert np.array_equal(got.triangles, r.triangles), q
+    b.close()
+
+
+@pytest.mark.slow
+def test_batch_gyroid_512_sixteen_isovalues_against_oracle(oracle_mod):
+    """The 512^3 gyroid (with noise, seed 0) at 16 isovalues evenly spaced in [-1, 1] as one
+    batch over the one resident volume; every job's mesh bit-exact against the oracle."""
+    import torch
+
+    from miniisosurface_amd import DeviceVolume, synth
+    from miniisosurface_amd.batch import BatchExtraction
+    g = synth.gyroid(512, seed=0, device="cuda")
+    dv = DeviceVolume.from_tensor(g)
+    isos = [float(x) for x in np.linspace(-1.0, 1.0, 16)]
+    b = BatchExtraction([(dv, i) for i in isos])
+    b.autotune(reps=1)
+    b.run()
+    torch.cuda.synchronize()
+    host = g.cpu().numpy()
+    print(f"[batch gyroid16] tuning {b.tuning()}, totals {b.totals()}", flush=True)
+    for q, iso in enumerate(isos):
+        r = oracle_mod.FEResult(host, iso, nthreads=_threads(), want_debug=False)
+        got = b.mesh(q)
+        print(f"  iso {iso:+.4f}: V={r.nverts} T={r.ntris}", flush=True)
+        assert got.points.shape[0] == r.nverts and got.triangles.shape[0] == r.ntris, q
+        assert bits_equal(got.points, r.points), q
+        assert bits_equal(got.normals, r.normals), q
+        assert np.array_equal(got.triangles, r.triangles), q
+    b.close()

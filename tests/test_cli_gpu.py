@@ -159,3 +159,73 @@ def test_reference_driver_bound_to_library(oracle_mod, name):
                                capture_output=True, text=True, timeout=300)
             assert r.returncode == 0, r.stderr
         assert open(os.path.join(tmp, "bound.vtk"), "rb").read() == open(os.path.join(tmp, "ref.vtk"), "rb").read()
+
+
+@pytest.mark.parametrize("name", ["rand20_iso050", "spacing_origin_24x19x14", "c1_gaussians_128"])
+def test_reference_mc_driver_bound_to_library(oracle_mod, name):
+    """The reference's own serial Marching Cubes driver, compiled with its MarchingCubes template
+    renamed out of the way and include/miniiso_mc_image.hpp supplying MarchingCubes(image,
+    isoval) over the library (oracle/Makefile target `integration`), writes the same mesh file
+    as the reference built as is."""
+    bound = oracle_mod.ref_binary("mcMiniiso")
+    ref = oracle_mod.ref_binary("mcSerial")
+    if not (bound and ref):
+        pytest.skip("reference drivers not built (no reference sources where the tree was built)")
+    from miniisosurface_amd import Image3D, save_image
+    p = [g for g in golden_files() if name in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        for exe, o in ((bound, "bound.vtk"), (ref, "ref.vtk")):
+            r = subprocess.run([exe, "-i", "v.vtk", "-o", o, "-v", repr(float(iso)), "-y", f"y_{o}/r.yaml"], cwd=tmp,
+                               capture_output=True, text=True, timeout=300)
+            assert r.returncode == 0, r.stderr
+            assert "Mini-Application Name: Marching Cubes" in r.stdout
+        assert open(os.path.join(tmp, "bound.vtk"), "rb").read() == open(os.path.join(tmp, "ref.vtk"), "rb").read()
+
+
+@pytest.mark.parametrize("name,sections,threads", [("c1_gaussians_128", (2, 2, 4), 4),
+                                                   ("rand20_iso050", (1, 1, 3), 3),
+                                                   ("spacing_origin_24x19x14", (2, 3, 2), 2),
+                                                   ("noncubic_33x17x9", (3, 1, 2), 6)])
+def test_mc_openmp_reference_same_contents(oracle_mod, name, sections, threads):
+    """The OpenMP Marching Cubes (marchingCubes/openmp/main.cpp:159-296): each section keeps its
+    own first-visit map and the section meshes are merged under `omp critical` in whatever order
+    the threads reach it (:271-290), so a vertex on a section seam appears once per section
+    and the numbering is not deterministic. Its own equivalence rule
+    (utilities/tests/SameContentsCheck.cpp:190-335) is what relates it to the serial order the
+    library returns: the library's mesh (the shim's sectioned overload) and the reference
+    driver's, run with several sections on several threads, are equivalent in both directions,
+    by the reference's checker and by ours; triangle counts are equal and the reference's extra
+    vertices are seam duplicates of the library's."""
+    exe = oracle_mod.ref_binary("mcOpenMP")
+    chk = oracle_mod.ref_binary("SameContentsCheck")
+    if not (exe and chk):
+        pytest.skip("reference drivers not built (no reference sources where the tree was built)")
+    import miniisosurface_amd as m
+    from miniisosurface_amd import Image3D, load_mesh, save_image, save_mesh
+    p = [g for g in golden_files() if name in g][0]
+    vol, iso, sp, org, d = load_fixture(p)
+    sx, sy, sz = sections
+    ours = m.MarchingCubes(m.Image3D(vol, sp, org), iso, sx, sy, sz)
+    with tempfile.TemporaryDirectory() as tmp:
+        save_image(os.path.join(tmp, "v.vtk"), Image3D(vol, sp, org))
+        save_mesh(os.path.join(tmp, "ours.vtk"), ours)
+        r = subprocess.run([exe, "-i", "v.vtk", "-o", "omp.vtk", "-v", repr(float(iso)), "-sx", str(sx), "-sy", str(sy),
+                            "-sz", str(sz), "-y", "y/r.yaml"], cwd=tmp, capture_output=True, text=True, timeout=300,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+        assert r.returncode == 0, r.stderr
+        assert f"Number of sections: {sx * sy * sz}" in r.stdout
+        omp = load_mesh(os.path.join(tmp, "omp.vtk"))
+        for a, b in (("ours.vtk", "omp.vtk"), ("omp.vtk", "ours.vtk")):
+            for tool in (chk, _tool("miniiso_same_contents")):
+                out = subprocess.run([tool, a, b], cwd=tmp, capture_output=True, text=True, timeout=120)
+                assert "The two meshes are equivalent." in out.stdout, (tool, a, b, out.stdout)
+    assert omp.triangles.shape == ours.triangles.shape
+    nv_ours, nv_omp = ours.points.shape[0], omp.points.shape[0]
+    assert nv_omp >= nv_ours
+    # the reference's vertices, duplicates removed, are exactly ours
+    key = lambda a: np.unique(np.ascontiguousarray(a).view(np.dtype((np.void, 12))).ravel())
+    assert np.array_equal(key(omp.points), key(ours.points))
+    print(f"[mc openmp] {name} sections {sections}: {nv_omp} vertices ({nv_omp - nv_ours} seam duplicates), "
+          f"{omp.triangles.shape[0]} triangles", flush=True)

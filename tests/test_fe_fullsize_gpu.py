@@ -146,3 +146,32 @@ def test_fe_1024_dense_slab_bit_exact(oracle_mod):
     gtri = mesh.triangles[gt0:gt1].astype(np.int64) - gv0
     stri = ref.triangles[st0:st1].astype(np.int64) - sv0
     assert np.array_equal(gtri, stri)
+
+
+def test_fe_1024_dense_whole_mesh_bit_exact(oracle_mod):
+    """The benchmark volume at iso 0.5, the whole mesh: all 39,458,009 vertices (points and
+    normals) and 78,770,672 triangles of the 1024^3 extraction bit for bit against the oracle's
+    OpenMP Flying Edges on the same volume (the serial reference's output, with its static row
+    loops and blocked scan, flyingEdges/openmp/FlyingEdgesAlgorithm.cpp:24-565)."""
+    import os
+
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+    from helpers import bits_equal
+
+    n, iso = 1024, 0.5
+    vol = synth.value_noise(n, seed=0, device="cuda")
+    algo = m.FlyingEdgesAlgorithm(m.DeviceVolume.from_tensor(vol), iso)
+    algo.run()
+    mesh = algo.moveOutput()
+    host = vol.cpu().numpy()
+    del algo, vol
+    print(f"[dense whole] GPU V={mesh.numberOfVertices()} T={mesh.numberOfTriangles()}", flush=True)
+    ref = oracle_mod.FEResult(host, iso, nthreads=min(16, os.cpu_count() or 1), want_debug=False)
+    del host
+    print(f"[dense whole] oracle V={ref.nverts} T={ref.ntris}", flush=True)
+    assert (ref.nverts, ref.ntris) == (39458009, 78770672)
+    assert mesh.numberOfVertices() == ref.nverts and mesh.numberOfTriangles() == ref.ntris
+    assert np.array_equal(mesh.triangles, ref.triangles)
+    assert bits_equal(mesh.points, ref.points)
+    assert bits_equal(mesh.normals, ref.normals)

@@ -101,3 +101,25 @@ def test_mc_order_switch_on_one_handle(oracle_mod):
     with pytest.raises(ValueError):
         m._MC(m.Image3D(vol), 0.5, order="sorted")
     assert m.lib().miso_mc_set_order(mc._h, 2) != 0            # the C ABI rejects other values
+
+
+@pytest.mark.slow
+def test_mc_c2_gyroid_512_both_orders(oracle_mod):
+    """The 512^3 gyroid with noise (seed 0) at isovalue 0: Marching Cubes in the serial
+    reference's first-visit order and in the duplicate-free variant's edge order, each mesh bit
+    for bit against the oracle (V = 10,804,576 and T = 21,539,597, as the reference gives)."""
+    import miniisosurface_amd as m
+    from miniisosurface_amd import synth
+    vol = synth.gyroid(512, 0)
+    ref = oracle_mod.MCResult(vol, 0.0)
+    assert (ref.nverts, ref.ntris) == (10804576, 21539597)
+    mc = m._MC(m.Image3D(vol), 0.0)
+    mc.run()
+    print("[mc 512] first visit", flush=True)
+    assert_mesh_equal(mc.mesh(), ref.points, ref.normals, ref.triangles, "C2 first visit")
+    p, n, t = ref.edge_order()
+    del ref
+    mc.set_order("edge")
+    mc.run()
+    print("[mc 512] edge order", flush=True)
+    assert_mesh_equal(mc.mesh(), p, n, t, "C2 edge order")
