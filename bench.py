@@ -1,17 +1,17 @@
 """Benchmark: Flying Edges on one MI355X at 1024^3 float32 (3-octave value noise, iso 0.5).
 
 Contract: `python bench.py [--gpus 1] [--steps K] [--warmup W]` prints ONE JSON line.
-A step is one complete extraction (passes 1-4: classify, count, scan, emit, with the host
-synchronisation after the scan that sizes the output) of the volume, already resident in HBM.
-value = cells / (wall time per step), in Gcells/s. The roofline object reports the extraction's
-compulsory HBM bytes, B_alg = 4N + 24V + 12T (the volume read once, points and normals, 32-bit
-triangle indices), over the summed device time of its four kernels, measured with HIP event
-pairs that the library records around each kernel on the stream the kernels run on (steps run
-before the timed ones; each event pair adds a few microseconds between the kernels, so the
-timed steps carry none but pass 4's). `roofline.kernels` gives each kernel its own compulsory
-bytes (pass 1: the volume and the sign mask; pass 2: mask and row counts; pass 3: row state;
-pass 4: points, normals, triangles, the mask and the row state it reads), its time and rate,
-and the HBM bytes its rocprofv3 counters show per launch.
+A step is one complete extraction (passes 1-4: classify, count, scan, emit, and a check of the
+totals against the output buffers) of the volume, already resident in HBM, by miso_fe_run on a
+handle that has its output buffers from an earlier step and no pass timing. value = cells /
+(wall time per step), in Gcells/s. The roofline object reports the extraction's compulsory HBM
+bytes, B_alg = 4N + 24V + 12T (the volume read once, points and normals, 32-bit triangle
+indices), over its device time per step, measured with a HIP event pair on the stream the
+library runs on (a torch stream handed to the handle) around the timed steps, and the HBM bytes
+the four kernels' rocprofv3 counters show. `roofline.kernels` gives each kernel, from steps run
+before the timed ones with every pass timed, its own compulsory bytes (pass 1: the volume and
+the sign mask; pass 2: mask and row counts; pass 3: row state; pass 4: points, normals,
+triangles, the mask and the row state it reads), its time and rate, and its counter traffic.
 """
 from __future__ import annotations
 
@@ -194,6 +194,35 @@ def summarize_pmc(fetch_dir, write_dir, out_path, command):
         wb, wr, wf = d.get("WRITE_SIZE", (0, 0, 0))
         lines.append(f"{key}: fetch_bytes={fb:.0f} write_bytes={wb:.0f} traffic_bytes={fb + wb:.0f} "
                      f"(raw fetch {fr:.0f} x {ff:.3f}, raw write {wr:.0f} x {wf:.3f})")
+    open(out_path, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def _csv_rows(path, suffix):
+    import csv
+    for dirpath, _, files in os.walk(path):
+        for f in files:
+            if f.endswith(suffix):
+                yield from csv.DictReader(open(os.path.join(dirpath, f)))
+
+
+def summarize_trace(trace_dir, out_path, command, title):
+    """Turns a `rocprofv3 --kernel-trace --output-format csv` run of this benchmark into a short
+    text summary: per kernel, calls and average / min / max duration (from the kernel trace)."""
+    rows = list(_csv_rows(trace_dir, "kernel_trace.csv"))
+    if not rows:
+        raise SystemExit(f"no kernel trace in {trace_dir}")
+    per = {}
+    for r in rows:
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        per.setdefault(r["Kernel_Name"], []).append(d)
+    lines = [f"# {title}", f"# command: {command}",
+             "# per kernel: calls, average / min / max duration in us (kernel trace), share of GPU time"]
+    total = sum(sum(v) for v in per.values())
+    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        short = name if len(name) <= 90 else name[:87] + "..."
+        lines.append(f"{short:<90} calls={len(v):5d} avg_us={sum(v) / len(v):9.1f} min_us={min(v):9.1f} "
+                     f"max_us={max(v):9.1f} pct={100 * sum(v) / total:6.2f}")
     open(out_path, "w").write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
@@ -557,9 +586,15 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "OUT"),
                     help="write the per-kernel HBM traffic summary of two rocprofv3 --pmc runs and exit")
     ap.add_argument("--pmc-command", default="", help="the profiled command, recorded in the summary")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_DIR", "OUT"),
+                    help="write the per-kernel summary of a rocprofv3 --kernel-trace run and exit")
+    ap.add_argument("--trace-title", default="", help="first line of the --summarize-trace summary")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, command=args.pmc_command)
+        return
+    if args.summarize_trace:
+        summarize_trace(*args.summarize_trace, command=args.pmc_command, title=args.trace_title)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (volumes would shard by z-slabs)",
@@ -577,7 +612,8 @@ def main():
         return
     n = args.n
     torch.cuda.init()
-    stream = torch.cuda.current_stream()
+    # the stream the library's kernels run on, and the events that time them (default priority)
+    stream = torch.cuda.Stream()
     t0 = time.time()
     vol = synth.value_noise(n, seed=0, device="cuda")
     torch.cuda.synchronize()
@@ -605,15 +641,20 @@ def main():
     algo.set_timing((1, 2, 3, 4))
     run_steps(args.warmup)
     pass_ms = run_steps(args.steps)
-    # the timed region: only pass 4, the dominant kernel whose duration the roofline needs,
-    # carries events; passes 1-3 run back to back as in a caller that reads no pass times
-    algo.set_timing((4,))
+    # the timed region: no event pairs between the kernels, timed on the host and with a HIP
+    # event pair around the steps on the stream the library runs on
+    algo.set_timing(())
     run_steps(1)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    timed_ms = run_steps(args.steps)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        algo.run()
+    ev1.record(stream)
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t1) / args.steps
+    dev_ms = ev0.elapsed_time(ev1) / args.steps
     V, T = algo.counts()
     _, _, _, bits, _, _ = algo.device_output()
 
@@ -632,7 +673,7 @@ def main():
         "miso::k_fe_scan": {"ms": pass_ms[2], "bytes": 16 * R + 16 * R},
         # points, normals, triangles written; the mask, counts and offsets read. Its reads of
         # the volume near the surface are not compulsory: pass 1 has read it once already
-        "miso::k_fe_emit_rows": {"ms": timed_ms[3], "bytes": 24 * V + tri_b + mask_b + 32 * R},
+        "miso::k_fe_emit_rows": {"ms": pass_ms[3], "bytes": 24 * V + tri_b + mask_b + 32 * R},
     }
     kern_total = float(pass_ms.sum())
     traffic_total = 0.0
@@ -647,7 +688,7 @@ def main():
                         "achieved_gb_s": round(d["bytes"] / (d["ms"] * 1e-3) / 1e9, 1),
                         "frac": round(d["bytes"] / (d["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                         "traffic_bytes": None if tr is None else int(tr)})
-    achieved = b_alg / (kern_total * 1e-3) / 1e9
+    achieved = b_alg / (dev_ms * 1e-3) / 1e9
     copy_gbs = stream_copy_gbs()
 
     # the sparse isovalue of the same config
@@ -681,18 +722,19 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"flying edges, {n}^3 float32 3-octave value noise (seed 0), iso {args.iso}",
                    "nverts": V, "ntris": T, "index_bits": bits},
-        "roofline": {"bound": "hbm", "kernel": "extraction: " + " + ".join(k["kernel"] for k in kernels),
+        "roofline": {"bound": "hbm",
+                     "kernel": "the whole extraction: k_fe_classify, k_fe_count_rows, k_fe_scan, k_fe_emit_rows",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if traffic_total is None else int(traffic_total),
-                     "alg_bytes": int(b_alg), "kernel_ms": round(kern_total, 4),
+                     "alg_bytes": int(b_alg), "extraction_ms_events": round(dev_ms, 4),
                      "traffic_over_alg": None if traffic_total is None else round(traffic_total / b_alg, 3),
                      "kernels": kernels},
-        "extraction": {"bytes_alg": b_alg, "kernel_ms": round(kern_total, 4),
+        "extraction": {"bytes_alg": b_alg, "device_ms_events": round(dev_ms, 4),
+                       "kernel_ms": round(kern_total, 4),
                        "hbm_frac_kernels": round(b_alg / (kern_total * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "hbm_frac_wall": round(b_alg / wall / 1e9 / HBM_PEAK_GBS, 4),
                        "pass_ms": [round(float(x), 4) for x in pass_ms],
-                       "pass4_ms_timed_steps": round(float(timed_ms[3]), 4),
                        "stream_copy_gb_s": round(copy_gbs, 1),
                        "frac_of_stream_copy_wall": round(b_alg / wall / 1e9 / copy_gbs, 4)},
         "sparse_iso_0.95": sparse,

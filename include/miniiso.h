@@ -88,11 +88,11 @@ int miso_fe_set_index_bits(miso_fe* h, int index_bits);
  * pass that miso_fe_set_timing leaves untimed. */
 int miso_fe_pass_times(const miso_fe* h, float ms[4]);
 
-/* Which passes the next runs time: bit p - 1 for pass p (default 0xF, all four). Each timed pass
+/* Which passes the next runs time: bit p - 1 for pass p (default 0, none). Each timed pass
  * records a pair of HIP events on the handle's stream, about 4-10 us of device time per pass
- * between the kernels, so callers that do not read the pass times can set 0 (or time only the
- * pass they look at). No counterpart in the reference, whose drivers time on the host. */
+ * between the kernels. No counterpart in the reference, whose drivers time on the host. */
 int miso_fe_set_timing(miso_fe* h, unsigned passes);
+
 
 /* Per grid row (ny*nz entries, r = j + ny*k): xl, xr (pass 1 trims), the cut x/y/z edge counts
  * (pass 2, before the scan) and the vertex offset (after pass 3); per cell row, stored at the

@@ -513,7 +513,7 @@ struct miso_fe {
     int done = 0;                    // last completed pass
     hipEvent_t ev[8] = {};           // start/stop of passes 1..4
     bool events = false;
-    unsigned timed = 0xFu;           // bit p - 1: pass p's device time is recorded with events
+    unsigned timed = 0u;             // bit p - 1: pass p's device time is recorded with events
     float pass_ms[4] = {0, 0, 0, 0};
 };
 

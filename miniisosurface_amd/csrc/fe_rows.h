@@ -465,20 +465,32 @@ __device__ __forceinline__ void emit_vertex(const float* __restrict__ plane, con
 constexpr int kEmitVertCap = 512;   // vertex-list entries per wavefront and round
 constexpr int kEmitCellCap = 512;   // active-cell entries per wavefront and round
 constexpr int kEmitTriCap = 64 * 5; // triangles of one group of 64 cells (at most 5 each)
+constexpr int kStageVerts = 320;    // vertices staged per wavefront and output window
 
-/* LDS of one wavefront: the vertex list, then (reusing it) the cell list and triangle map, beside
- * the 8 edge-combination cut words and id bases of each lane. */
+/* LDS of one wavefront. The triangles use the 8 edge-combination cut words and id bases of each
+ * lane, the cell list and the triangle map; the vertices before them use the same bytes for their
+ * list and staging window (sv), or, without staging, the vertex list (u.vlist). */
 struct EmitRowsLds {
-    uint64_t cut[8][64];
-    uint32_t base[8][64];
     union {
-        uint32_t vlist[kEmitVertCap];
         struct {
-            uint32_t cells[kEmitCellCap];
-            uint32_t map[kEmitTriCap];
-        } t;
-    } u;
+            uint64_t cut[8][64];
+            uint32_t base[8][64];
+            union {
+                uint32_t vlist[kEmitVertCap];
+                struct {
+                    uint32_t cells[kEmitCellCap];
+                    uint32_t map[kEmitTriCap];
+                } t;
+            } u;
+        };
+        struct {
+            uint32_t list[kStageVerts];
+            float stage[6 * kStageVerts];   // points, then normals, of the window's vertices
+        } sv;
+    };
 };
+static_assert(sizeof(EmitRowsLds) == 8 * 64 * 8 + 8 * 64 * 4 + (kEmitCellCap + kEmitTriCap) * 4,
+              "the staging window fits in the triangles' bytes");
 
 /* case of the cell at bit b of a 32-bit half (corner bits v0..v7 as case_of) */
 __device__ __forceinline__ uint32_t case_of_half(const uint32_t h[8], int b)
@@ -637,6 +649,108 @@ __device__ __forceinline__ void emit_rows_vertices(const EmitRows& E, uint32_t* 
     }
 }
 
+/* Vertices of the wavefront's rows through an LDS staging window. The wavefront's vertices form
+ * one contiguous output range [v0, v0 + total), written kStageVerts vertices at a time: in each
+ * window the lanes list their cut edges whose output position falls in it (x-edges in lane order,
+ * y/z-edges word-major for rows of 16 words and more, the order in which the gathers share cache
+ * lines best), compute one vertex each, and put its point and normal at the vertex's place in the
+ * staging buffer; the window then goes out as consecutive dwords, every store of the wavefront
+ * covering 256 contiguous bytes. Stored straight from the lanes, a wavefront's vertices land in
+ * runs of a few vertices (one row's cut edges of one axis, or fewer in word-major order), and
+ * cache lines at the runs' ends are written partially, several times over. */
+template <bool Unit>
+__device__ __forceinline__ void emit_rows_vertices_staged(const EmitRows& E, EmitRowsLds& S,
+                                                          const float* __restrict__ vol, const Geom& g, float iso,
+                                                          float* __restrict__ pts, float* __restrict__ nrm)
+{
+    const RowLanes& rl = E.rl;
+    const LaneWords& L = E.L;
+    const int lane = E.lane;
+    const uint64_t v0 = E.v0, vb = E.vb, eb = E.eb;
+    const int j0 = E.j0;
+    const int64_t k0 = E.k0;
+    const float* __restrict__ plane = vol + k0 * (g.nx * g.ny);
+    const uint64_t live = E.emit ? ~0ull : 0ull;
+    const uint64_t cut[3] = {(L.m[0] ^ shifted(L.m[0], L.n[0])) & eb & live,
+                             E.has_y ? ((L.m[0] ^ L.m[1]) & vb & live) : 0ull,
+                             E.has_z ? ((L.m[0] ^ L.m[2]) & vb & live) : 0ull};
+    const uint32_t c0 = __popcll(cut[0]), c1 = __popcll(cut[1]), c2 = __popcll(cut[2]);
+    // output positions (wavefront-relative) of the lane's first x-, y- and z-edge vertex: rows in
+    // order, [x][y][z] within a row, words in order within an axis
+    const uint32_t p01 = c0 | c1 << 16;
+    const uint32_t inc01 = seg_incl<OpAdd>(p01, rl.Wp, rl.w);
+    const uint32_t tot01 = (uint32_t)__shfl((int)inc01, (lane & ~(rl.Wp - 1)) + rl.Wp - 1);
+    const uint32_t ex01 = inc01 - p01;
+    const uint32_t X = tot01 & 0xFFFFu, Y = tot01 >> 16;
+    const uint32_t lane_tot = c0 + c1 + c2;
+    const uint32_t incl = wave_incl_scan_u32(lane_tot);
+    const uint32_t before_row = __shfl(incl - lane_tot, rl.q * rl.Wp);
+    const uint32_t pos[3] = {before_row + (ex01 & 0xFFFFu), before_row + X + (ex01 >> 16),
+                             before_row + X + Y + seg_excl_scan(c2, rl.Wp, rl.w)};
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    const bool word_major = rl.Wp >= 16;
+    const int wp_log2 = __builtin_ctz((unsigned)rl.Wp), rpw_log2 = 6 - wp_log2;
+    const int tl = word_major ? rl.w * rl.rpw + rl.q : lane;
+    const int from = word_major ? ((lane & (rl.rpw - 1)) << wp_log2) + (lane >> rpw_log2) : lane;
+    uint32_t* list = S.sv.list;
+    float* stage = S.sv.stage;
+    for (uint32_t w0 = 0; w0 < total; w0 += kStageVerts) {
+        const uint32_t w1 = min(w0 + (uint32_t)kStageVerts, total);
+        uint32_t n[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t lo = max(pos[a], w0), hi = min(pos[a] + (a == 0 ? c0 : (a == 1 ? c1 : c2)), w1);
+            n[a] = hi > lo ? hi - lo : 0u;
+        }
+        // list slots: the window's x-edges [0, xtot) in lane order, then its y/z-edges
+        const uint32_t xincl = wave_incl_scan_u32(n[0]);
+        const uint32_t xtot = (uint32_t)__builtin_amdgcn_readlane((int)xincl, 63);
+        const uint32_t n12t = (uint32_t)__shfl((int)(n[1] + n[2]), from);
+        const uint32_t yzincl_t = wave_incl_scan_u32(n12t);
+        const uint32_t wn = (uint32_t)__builtin_amdgcn_readlane((int)yzincl_t, 63) + xtot;
+        uint32_t slot[3];
+        slot[0] = xincl - n[0];
+        slot[1] = xtot + (uint32_t)__shfl((int)(yzincl_t - n12t), tl);
+        slot[2] = slot[1] + n[1];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (!n[a]) continue;
+            uint64_t bits = cut[a];
+            uint32_t p = pos[a], s = slot[a];
+            while (bits && p < w1) {
+                const int b = __builtin_ctzll(bits);
+                bits &= bits - 1;
+                if (p >= w0) list[s++] = (p - w0) << 14 | (uint32_t)a << 12 | (uint32_t)lane << 6 | (uint32_t)b;
+                ++p;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int grp = 0; grp < 2; ++grp) {
+            for (uint32_t e = (grp ? xtot : 0u) + (uint32_t)lane; e < (grp ? wn : xtot); e += 64) {
+                const uint32_t ent = list[e];
+                float pp[3], nn[3];
+                emit_vertex<Unit>(plane, g, iso, ent, j0, (int)k0, wp_log2, grp == 0, pp, nn);
+                const uint32_t at = 3 * (ent >> 14);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    stage[at + c] = pp[c];
+                    stage[3 * kStageVerts + at + c] = nn[c];
+                }
+            }
+        }
+        wave_sync();
+        const uint32_t nf = 3 * (w1 - w0);
+        float* __restrict__ dp = pts + 3 * (v0 + w0);
+        float* __restrict__ dn = nrm + 3 * (v0 + w0);
+        for (uint32_t f = (uint32_t)lane; f < nf; f += 64) {
+            dp[f] = stage[f];
+            dn[f] = stage[3 * kStageVerts + f];
+        }
+        wave_sync();
+    }
+}
+
 /* Triangles of the wavefront's cell rows: active cells listed per lane in output order, then
  * resolved one triangle per lane. Ids are written less vbias, or through remap. */
 template <typename Index, bool Remap>
@@ -770,7 +884,7 @@ __device__ __forceinline__ void fe_emit_rows_block(const uint64_t* __restrict__ 
 {
     EmitRows E;
     if (!emit_rows_setup(E, mask, g, counts, voff, toff, blk)) return;
-    if (!Remap) emit_rows_vertices<Unit>(E, s_w[E.wv].u.vlist, vol, g, iso, pts, nrm);
+    if (!Remap) emit_rows_vertices_staged<Unit>(E, s_w[E.wv], vol, g, iso, pts, nrm);
     emit_rows_triangles<Index, Remap>(E, s_w[E.wv], s_case, g, counts, voff, tris, vbias, remap);
 }
 
