@@ -522,7 +522,7 @@ def bench_serving(n_requests, rate=200.0, seed=0, saturated_rate=20000.0):
     import torch
 
     from miniisosurface_amd import DeviceVolume, synth
-    from miniisosurface_amd.serve import IsosurfaceServer, latency_stats, poisson_requests
+    from miniisosurface_amd.serve import IsosurfaceServer, Request, latency_stats, poisson_requests
 
     pool = synth.serving_pool(16, seed=seed, device="cuda")
     dvs = [DeviceVolume.from_tensor(t) for t, _, _ in pool]
@@ -532,7 +532,11 @@ def bench_serving(n_requests, rate=200.0, seed=0, saturated_rate=20000.0):
 
     def play(deliver, reqs, on_done=None):
         srv = IsosurfaceServer(dvs, budget_bytes=200e9, max_jobs=64, deliver=deliver)
-        srv.run(poisson_requests(8, 1e4, ranges, seed=seed + 2), on_done=on_done)   # warm-up
+        # warm-up: kernels, slot buffers, and pinned host blocks of every size class the stream
+        # needs (one request per pool volume near the middle of its range, then a short burst)
+        srv.run([Request(volume=q, iso=float(np.float32((lo + hi) / 2)), arrival=0.0, rid=q)
+                 for q, (lo, hi) in enumerate(ranges)], on_done=on_done)
+        srv.run(poisson_requests(8, 1e4, ranges, seed=seed + 2), on_done=on_done)
         srv.steps, srv.step_sizes, srv.peak_live_bytes = 0, [], 0
         done = srv.run(reqs, on_done=on_done)
         st = latency_stats(done)
@@ -560,6 +564,9 @@ def bench_serving(n_requests, rate=200.0, seed=0, saturated_rate=20000.0):
     st["saturated"] = play("host", poisson_requests(2 * n_requests, saturated_rate, ranges, seed=seed + 3),
                            on_done=consume)
     st["saturated"]["offered_rate_per_s"] = saturated_rate
+    st["saturated_device_handoff"] = play("device", poisson_requests(2 * n_requests, saturated_rate, ranges,
+                                                                     seed=seed + 3), on_done=lambda r: r.release())
+    st["saturated_device_handoff"]["offered_rate_per_s"] = saturated_rate
     return st
 
 
