@@ -209,6 +209,24 @@ int miso_batch_emit_async(miso_batch* b, int64_t njobs_emit, float* points, floa
                           uint64_t cap_vertices, uint64_t cap_triangles);
 int miso_batch_wait(miso_batch* b);
 
+/* ---------------------------------------------------------------- Flying Edges without allocation */
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): miso_fe_run in one
+ * call on a device-resident volume, with no device allocation and no free. The row state of the
+ * four passes (sign mask, row counts and trims, offsets, scan state, totals) lives in a static
+ * device array of the library, 64 MiB per GPU, held from the library's load until the process
+ * ends; volumes whose row state exceeds it (one bit per voxel plus about 40 bytes per grid row:
+ * up to about 700^3 voxels) are refused with MISO_ERR_ARG, as are rows of more than 4096 voxels. The caller passes the output buffers: points and normals of cap_vertices * 3 floats,
+ * triangles of cap_triangles * 3 uint32 indices (each triangle indexes this mesh's vertices).
+ * Kernels run on `stream` (NULL: the null stream); the call returns when they are done, with the
+ * mesh's counts in *nverts and *ntris. If the mesh does not fit the buffers, nothing is written
+ * to them, the counts are still returned, and the status is MISO_ERR_ARG. Every call on one GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (from several threads or
+ * streams) race and must be serialised by the caller. Same output as miso_fe_run. */
+int miso_fe_run_noalloc(const miso_volume* vol, float isoval, void* stream, float* points, float* normals,
+                        uint32_t* triangles, uint64_t cap_vertices, uint64_t cap_triangles, uint64_t* nverts,
+                        uint64_t* ntris);
+
 /* ---------------------------------------------------------------- library */
 const char* miso_last_error(void);
 /* "miniiso <version> gfx950" */

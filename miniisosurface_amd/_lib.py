@@ -98,6 +98,8 @@ def lib():
         "miso_batch_job_offsets": [P, ctypes.c_int64, U64P, U64P],
         "miso_batch_emit_async": [P, ctypes.c_int64, P, P, P, ctypes.c_uint64, ctypes.c_uint64],
         "miso_batch_wait": [P],
+        "miso_fe_run_noalloc": [ctypes.POINTER(Volume), F, P, P, P, P, ctypes.c_uint64, ctypes.c_uint64, U64P,
+                                U64P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

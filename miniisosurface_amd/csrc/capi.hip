@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1195,6 +1196,128 @@ extern "C" void miso_mc_destroy(miso_mc* h)
     for (int e = 0; e < 2; ++e)
         if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
     delete h;
+}
+
+/* ====================================================================== Flying Edges without allocation */
+
+/* miso_fe_run_noalloc keeps all of its per-extraction device state -- the sign mask, the row counts
+ * and trims, the row offsets, the scan state and the two totals -- in this file-scope array instead
+ * of memory allocated per call. Each GPU holds its own copy from the library's load to the end of
+ * the process. In host code `scratch` names no device address; the device address is taken per
+ * device with hipGetSymbolAddress (scratch_ptr) and only that is passed to kernels. */
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+/* The device address of `scratch` on the current device, looked up on the first call there. */
+int scratch_ptr(void** out)
+{
+    int dev = 0;
+    HIPCALL(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(MISO_ERR_ARG, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCALL(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(MISO_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return MISO_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+/* Byte offsets of the row state inside the scratch array for geometry g; the total in `end`. */
+struct ScratchLayout {
+    size_t mask, counts, trims, voff, toff, scan, totals, end;
+    explicit ScratchLayout(const Geom& g)
+    {
+        const size_t R = (size_t)(g.ny * g.nz);
+        const size_t ntiles = (R + kScanTile - 1) / kScanTile;
+        mask = 0;
+        counts = align256(mask + R * (size_t)g.W * sizeof(uint64_t));
+        trims = align256(counts + R * sizeof(RowCounts));
+        voff = align256(trims + R * sizeof(RowTrim));
+        toff = align256(voff + R * sizeof(uint64_t));
+        scan = align256(toff + R * sizeof(uint64_t));
+        totals = align256(scan + (1 + 2 * ntiles) * sizeof(uint64_t));
+        end = totals + 2 * sizeof(uint64_t);
+    }
+};
+
+}  // namespace
+
+extern "C" int miso_fe_run_noalloc(const miso_volume* vol, float isoval, void* stream, float* points,
+                                   float* normals, uint32_t* triangles, uint64_t cap_vertices,
+                                   uint64_t cap_triangles, uint64_t* nverts, uint64_t* ntris)
+{
+    if (int rc = check_volume(vol)) return rc;
+    if (!vol->on_device) return fail(MISO_ERR_ARG, "miso_fe_run_noalloc takes a device-resident volume");
+    if (nverts) *nverts = 0;
+    if (ntris) *ntris = 0;
+    const Geom g = make_geom(vol);
+    if (!has_cells(g)) return MISO_OK;
+    RowState s;               // borrowed pointers only; nothing allocated, nothing released
+    s.g = g;
+    s.d_vol = vol->data;
+    if (!s.lane_rows())
+        return fail(MISO_ERR_ARG, "miso_fe_run_noalloc takes rows of at most 4096 voxels (nx = %lld)",
+                    (long long)g.nx);
+    const ScratchLayout lay(g);
+    if (lay.end > kScratchBytes)
+        return fail(MISO_ERR_ARG,
+                    "volume %lld x %lld x %lld needs %zu bytes of row state, above the %zu-byte scratch of "
+                    "miso_fe_run_noalloc",
+                    (long long)g.nx, (long long)g.ny, (long long)g.nz, lay.end, kScratchBytes);
+    void* base = nullptr;
+    if (int rc = scratch_ptr(&base)) return rc;
+    uint8_t* sp = (uint8_t*)base;
+    s.stream = (hipStream_t)stream;
+    s.d_mask = (uint64_t*)(sp + lay.mask);
+    s.d_counts = (RowCounts*)(sp + lay.counts);
+    s.d_trims = (RowTrim*)(sp + lay.trims);
+    s.d_voff = (uint64_t*)(sp + lay.voff);
+    s.d_toff = (uint64_t*)(sp + lay.toff);
+    s.d_scan = (uint64_t*)(sp + lay.scan);
+    s.d_totals = (uint64_t*)(sp + lay.totals);
+    s.ntiles = (g.ny * g.nz + kScanTile - 1) / kScanTile;
+    // passes 1-3 (pass 2 clears the scan state that pass 3 starts from); pass 4 checks the totals
+    // against the caller's capacities on the device and writes nothing unless they fit
+    if (int rc = s.launch_classify(isoval)) return rc;
+    if (int rc = s.launch_count()) return rc;
+    hipLaunchKernelGGL(k_fe_scan, dim3((unsigned)s.ntiles), dim3(256), 0, s.stream, (const RowCounts*)s.d_counts,
+                       g.ny * g.nz, s.d_voff, s.d_toff, s.d_scan, s.ntiles, s.d_totals);
+    HIPCALL(hipGetLastError());
+    {
+        const uint64_t* need = s.d_totals;
+        if (s.unit_geometry())
+            hipLaunchKernelGGL((k_fe_emit_rows<uint32_t, true>), dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                               s.d_vol, (const uint64_t*)s.d_mask, g, isoval, (const RowCounts*)s.d_counts,
+                               (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, points, normals, triangles,
+                               (uint64_t)0, need, cap_vertices, cap_triangles);
+        else
+            hipLaunchKernelGGL((k_fe_emit_rows<uint32_t, false>), dim3(s.lane_row_blocks()), dim3(256), 0, s.stream,
+                               s.d_vol, (const uint64_t*)s.d_mask, g, isoval, (const RowCounts*)s.d_counts,
+                               (const uint64_t*)s.d_voff, (const uint64_t*)s.d_toff, points, normals, triangles,
+                               (uint64_t)0, need, cap_vertices, cap_triangles);
+        HIPCALL(hipGetLastError());
+    }
+    uint64_t tot[2] = {0, 0};
+    HIPCALL(hipMemcpyAsync(tot, s.d_totals, sizeof(tot), hipMemcpyDeviceToHost, s.stream));
+    HIPCALL(hipStreamSynchronize(s.stream));
+    if (nverts) *nverts = tot[0];
+    if (ntris) *ntris = tot[1];
+    if (tot[0] > cap_vertices || tot[1] > cap_triangles)
+        return fail(MISO_ERR_ARG, "the mesh has %llu vertices and %llu triangles, above the buffers' %llu and %llu "
+                    "(nothing written; the counts are returned)", (unsigned long long)tot[0],
+                    (unsigned long long)tot[1], (unsigned long long)cap_vertices, (unsigned long long)cap_triangles);
+    return MISO_OK;
 }
 
 /* ====================================================================== library */
