@@ -187,7 +187,16 @@ int miso_batch_copy_out(miso_batch* b, int64_t job, float* points, float* normal
 /* launch shape of the persistent kernels: resident blocks per CU and work items per ticket */
 int miso_batch_set_tuning(miso_batch* b, int blocks_per_cu, int chunk);
 int miso_batch_get_tuning(const miso_batch* b, int* blocks_per_cu, int* chunk);
-/* times every shape of a small grid on this batch (reps runs each), keeps the fastest */
+/* Pass 4's work queue. DISPATCH (the default): one queue, row blocks in job order. XCD: one
+ * queue per XCD of the MI355X (a persistent block serves the queue of the XCD it runs on, block
+ * b on XCD b mod 8); each XCD owns every eighth row block of a job, in the tile order of the
+ * single-volume pass 4, so that neighbouring rows meet in one XCD's L2. Same output. */
+#define MISO_BATCH_QUEUE_DISPATCH 0
+#define MISO_BATCH_QUEUE_XCD 1
+int miso_batch_set_queue_order(miso_batch* b, int order);
+int miso_batch_get_queue_order(const miso_batch* b, int* order);
+/* times every setting of a small grid on this batch (reps runs each) -- resident blocks per CU,
+ * items per ticket, pass 4's queue order -- and keeps the fastest */
 int miso_batch_autotune(miso_batch* b, int reps, float* best_ms);
 /* device time of the last run, passes 1-4 */
 int miso_batch_time_ms(const miso_batch* b, float* ms);

@@ -78,6 +78,15 @@ class BatchExtraction:
         check(lib().miso_batch_get_tuning(self._h, ctypes.byref(b), ctypes.byref(c)), "get_tuning")
         return b.value, c.value
 
+    def set_queue_order(self, order):
+        """Pass 4's work queue: "dispatch" (one queue) or "xcd" (one per XCD, tiled rows)."""
+        check(lib().miso_batch_set_queue_order(self._h, {"dispatch": 0, "xcd": 1}[order]), "set_queue_order")
+
+    def queue_order(self):
+        o = ctypes.c_int()
+        check(lib().miso_batch_get_queue_order(self._h, ctypes.byref(o)), "get_queue_order")
+        return ("dispatch", "xcd")[o.value]
+
     def autotune(self, reps=3):
         ms = ctypes.c_float()
         check(lib().miso_batch_autotune(self._h, int(reps), ctypes.byref(ms)), "autotune")

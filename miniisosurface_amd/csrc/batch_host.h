@@ -26,7 +26,8 @@ struct miso_batch {
     RowTrim* d_trims = nullptr;
     uint64_t* d_voff = nullptr;
     uint64_t* d_toff = nullptr;
-    uint64_t* d_state = nullptr;      // [0..2] queue tickets, [3] totals V, [4] totals T, [5..] scan state
+    uint64_t* d_state = nullptr;      // [0..2] queue tickets, [3] totals V, [4] totals T, [5..12] the XCD
+                                      // queues' tickets of pass 4, [13..] scan state
     int64_t state_words = 0;
     uint64_t* d_jobout = nullptr;     // 4 per job: first vertex, first triangle, vertices, triangles
     uint64_t* h_jobout = nullptr;     // pinned copy, then the two totals
@@ -35,6 +36,10 @@ struct miso_batch {
     uint32_t* d_tris = nullptr;
     uint64_t cap_v = 0, cap_t = 0;
     int blocks_per_cu = 4, chunk = 4, num_cus = 256;
+    int queue_order = MISO_BATCH_QUEUE_DISPATCH;     // pass 4's work queue: one, or one per XCD
+    std::vector<int64_t> xcd_prefix;                 // 8 x (njobs + 1): XCD x's pass 4 items before job j
+    int64_t* d_xcd_prefix = nullptr;
+    int64_t cap_xcd = 0;
     int64_t cap_jobs = 0, cap_prefix_c = 0, cap_prefix_l = 0, cap_mask = 0, cap_counts = 0, cap_trims = 0;
     int64_t cap_voff = 0, cap_toff = 0, cap_state = 0, cap_jobout = 0, cap_host = 0, cap_groups = 0;
     bool counted = false;             // split-phase use: counts of the current job set are on the host
@@ -45,7 +50,8 @@ struct miso_batch {
 
     uint64_t* tickets() { return d_state; }
     uint64_t* totals() { return d_state + 3; }
-    uint64_t* scan_state() { return d_state + 5; }
+    uint64_t* xcd_tickets() { return d_state + 5; }
+    uint64_t* scan_state() { return d_state + 13; }
 };
 
 namespace {
@@ -115,13 +121,18 @@ int enqueue_emit_into(miso_batch* b, int64_t nemit, float* pts, float* nrm, uint
     const int64_t nlane = b->lane_prefix[nemit];
     if (nlane == 0) return MISO_OK;
     const uint64_t* need = nemit < b->njobs ? b->d_jobout + 4 * nemit : b->totals();
-    // the emit queue's ticket starts at zero on every emit, also a second emit after one count
+    // the emit queue's tickets start at zero on every emit, also a second emit after one count
     // (a fill kernel, not a memset node, so that the enqueue can be captured)
-    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, b->stream, b->tickets() + 2, (int64_t)1);
-    hipLaunchKernelGGL(k_batch_emit, dim3(batch_grid(b, nlane)), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
+    const bool xcd = b->queue_order == MISO_BATCH_QUEUE_XCD;
+    uint64_t* tk = xcd ? b->xcd_tickets() : b->tickets() + 2;
+    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, b->stream, tk, (int64_t)(xcd ? 8 : 1));
+    unsigned grid = batch_grid(b, nlane);
+    if (xcd) grid = std::max(8u, (grid + 7u) / 8u * 8u);     // every XCD queue has blocks
+    hipLaunchKernelGGL(k_batch_emit, dim3(grid), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
                        (const int64_t*)b->d_lane_prefix, b->njobs, nlane, b->chunk, (const uint64_t*)b->d_mask,
                        (const RowCounts*)b->d_counts, (const uint64_t*)b->d_voff, (const uint64_t*)b->d_toff,
-                       (const uint64_t*)b->d_jobout, need, cap_v, cap_t, pts, nrm, tris, b->tickets() + 2);
+                       (const uint64_t*)b->d_jobout, need, cap_v, cap_t, pts, nrm, tris, tk,
+                       xcd ? (const int64_t*)b->d_xcd_prefix : (const int64_t*)nullptr, (int)nemit);
     HIPCALL(hipGetLastError());
     return MISO_OK;
 }
@@ -187,6 +198,7 @@ void batch_release(miso_batch* b)
     dev_free(b->d_cls_prefix);
     dev_free(b->d_group_job);
     dev_free(b->d_lane_prefix);
+    dev_free(b->d_xcd_prefix);
     dev_free(b->d_mask);
     dev_free(b->d_counts);
     dev_free(b->d_trims);
@@ -290,12 +302,20 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
     b->R = R;
     b->mask_words = mw;
     b->ntiles = (R + kScanTile - 1) / kScanTile;
-    b->state_words = 5 + 1 + 2 * b->ntiles;
+    b->state_words = 13 + 1 + 2 * b->ntiles;
+    // pass 4's XCD queues: XCD x owns the dispatch indices 8 t + x of each job's row blocks
+    b->xcd_prefix.assign(8 * (size_t)(njobs + 1), 0);
+    for (int x = 0; x < 8; ++x)
+        for (int64_t j = 0; j < njobs; ++j) {
+            const int64_t n = b->lane_prefix[j + 1] - b->lane_prefix[j];
+            b->xcd_prefix[x * (njobs + 1) + j + 1] = b->xcd_prefix[x * (njobs + 1) + j] + (n > x ? (n - x + 7) / 8 : 0);
+        }
     int rc = 0;
     if ((rc = grow(&b->d_jobs, &b->cap_jobs, njobs))) return rc;
     if ((rc = grow(&b->d_cls_prefix, &b->cap_prefix_c, njobs + 1))) return rc;
     if ((rc = grow(&b->d_group_job, &b->cap_groups, njobs + 1))) return rc;
     if ((rc = grow(&b->d_lane_prefix, &b->cap_prefix_l, njobs + 1))) return rc;
+    if ((rc = grow(&b->d_xcd_prefix, &b->cap_xcd, 8 * (njobs + 1)))) return rc;
     if ((rc = grow(&b->d_mask, &b->cap_mask, mw))) return rc;
     if ((rc = grow(&b->d_counts, &b->cap_counts, R))) return rc;
     if ((rc = grow(&b->d_trims, &b->cap_trims, R))) return rc;
@@ -316,6 +336,8 @@ int batch_set_jobs(miso_batch* b, const miso_volume* vols, const float* isovals,
     HIPCALL(hipMemcpyAsync(b->d_group_job, b->group_job.data(), (b->ngroups + 1) * sizeof(int), hipMemcpyHostToDevice,
                            b->stream));
     HIPCALL(hipMemcpyAsync(b->d_lane_prefix, b->lane_prefix.data(), (njobs + 1) * sizeof(int64_t),
+                           hipMemcpyHostToDevice, b->stream));
+    HIPCALL(hipMemcpyAsync(b->d_xcd_prefix, b->xcd_prefix.data(), 8 * (njobs + 1) * sizeof(int64_t),
                            hipMemcpyHostToDevice, b->stream));
     HIPCALL(hipStreamSynchronize(b->stream));
     return MISO_OK;
@@ -449,6 +471,30 @@ extern "C" int miso_batch_set_tuning(miso_batch* b, int blocks_per_cu, int chunk
     return MISO_OK;
 }
 
+extern "C" int miso_batch_set_queue_order(miso_batch* b, int order)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    if (order != MISO_BATCH_QUEUE_DISPATCH && order != MISO_BATCH_QUEUE_XCD)
+        return fail(MISO_ERR_ARG, "queue order is MISO_BATCH_QUEUE_DISPATCH or MISO_BATCH_QUEUE_XCD");
+    if (order == b->queue_order) return MISO_OK;
+    HIPCALL(hipStreamSynchronize(b->stream));
+    b->queue_order = order;
+    if (b->g_emit || b->g_all) {
+        if (b->g_emit) (void)hipGraphExecDestroy(b->g_emit);
+        if (b->g_all) (void)hipGraphExecDestroy(b->g_all);
+        b->g_emit = b->g_all = nullptr;
+        if (b->d_pts) return batch_capture_emit(b);
+    }
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_get_queue_order(const miso_batch* b, int* order)
+{
+    if (!b || !order) return fail(MISO_ERR_ARG, "NULL argument");
+    *order = b->queue_order;
+    return MISO_OK;
+}
+
 extern "C" int miso_batch_get_tuning(const miso_batch* b, int* blocks_per_cu, int* chunk)
 {
     if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
@@ -463,25 +509,30 @@ extern "C" int miso_batch_autotune(miso_batch* b, int reps, float* best_ms)
     if (reps < 1) reps = 1;
     static const int kBlocks[] = {2, 4, 8};
     static const int kChunk[] = {1, 4, 16};
+    static const int kOrder[] = {MISO_BATCH_QUEUE_DISPATCH, MISO_BATCH_QUEUE_XCD};
     float best = -1.0f;
-    int bb = b->blocks_per_cu, bc = b->chunk;
-    for (int x : kBlocks)
-        for (int c : kChunk) {
-            if (int rc = miso_batch_set_tuning(b, x, c)) return rc;
-            if (int rc = batch_run(b)) return rc;   // warm: sizes the output, captures the graphs
-            float sum = 0.0f;
-            for (int r = 0; r < reps; ++r) {
-                if (int rc = batch_run(b)) return rc;
-                sum += b->last_ms;
+    int bb = b->blocks_per_cu, bc = b->chunk, bo = b->queue_order;
+    for (int o : kOrder)
+        for (int x : kBlocks)
+            for (int c : kChunk) {
+                if (int rc = miso_batch_set_tuning(b, x, c)) return rc;
+                if (int rc = miso_batch_set_queue_order(b, o)) return rc;
+                if (int rc = batch_run(b)) return rc;   // warm: sizes the output, captures the graphs
+                float sum = 0.0f;
+                for (int r = 0; r < reps; ++r) {
+                    if (int rc = batch_run(b)) return rc;
+                    sum += b->last_ms;
+                }
+                const float ms = sum / reps;
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    bb = x;
+                    bc = c;
+                    bo = o;
+                }
             }
-            const float ms = sum / reps;
-            if (best < 0.0f || ms < best) {
-                best = ms;
-                bb = x;
-                bc = c;
-            }
-        }
     if (int rc = miso_batch_set_tuning(b, bb, bc)) return rc;
+    if (int rc = miso_batch_set_queue_order(b, bo)) return rc;
     if (best_ms) *best_ms = best;
     return MISO_OK;
 }

@@ -76,6 +76,43 @@ __device__ __forceinline__ void run_queue(uint64_t* __restrict__ ticket, const i
     }
 }
 
+/* The XCD-aware variant of run_queue for pass 4. A persistent block b runs on XCD b mod 8 (the
+ * dispatcher hands workgroups to the 8 XCDs in turn), and each XCD takes its items from a queue of
+ * its own (ticket x): within job j, XCD x owns the dispatch indices b = 8 t + x < n_j, which
+ * emit_block_order maps to the job's row blocks as it maps the single-volume kernel's blocks, so
+ * each XCD's L2 sees tiles of neighbouring rows, and all XCDs move through a job's planes
+ * together. xprefix[x stride + j]: XCD x's items before job j (jobs [0, njobs) are drawn from).
+ * body(job, dispatch index). */
+template <class Body>
+__device__ __forceinline__ void run_queue_xcd(uint64_t* __restrict__ tickets, const int64_t* __restrict__ xprefix,
+                                              int64_t stride, int njobs, int chunk, Body body)
+{
+    __shared__ int64_t s_base;
+    __shared__ int s_job;
+    const int x = (int)(blockIdx.x & 7u);
+    const int64_t* __restrict__ pre = xprefix + (int64_t)x * stride;
+    const int64_t nitems = pre[njobs];
+    for (;;) {
+        __syncthreads();   // every thread has read the previous ticket
+        if (threadIdx.x == 0) {
+            const int64_t b = (int64_t)__hip_atomic_fetch_add((unsigned long long*)&tickets[x],
+                                                              (unsigned long long)chunk, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT);
+            s_base = b;
+            s_job = b < nitems ? find_job(pre, njobs, b) : 0;
+        }
+        __syncthreads();
+        const int64_t base = s_base;
+        if (base >= nitems) break;
+        int j = __builtin_amdgcn_readfirstlane(s_job);
+        const int64_t end = base + chunk < nitems ? base + chunk : nitems;
+        for (int64_t it = base; it < end; ++it) {
+            while (it >= pre[j + 1]) ++j;
+            body(j, 8 * (it - pre[j]) + x);
+        }
+    }
+}
+
 /* clears the queue tickets and the scan's tile status (a kernel, so it can sit in a graph) */
 __global__ __launch_bounds__(256) void k_batch_zero(uint64_t* __restrict__ words, int64_t n)
 {
@@ -139,7 +176,9 @@ __global__ __launch_bounds__(256) void k_batch_job_counts(const BatchJob* __rest
 }
 
 /* Pass 4 for the first jobs (items [0, nitems)); writes nothing when their output exceeds the
- * capacity (the host then grows the buffers and runs this pass again). */
+ * capacity (the host then grows the buffers and runs this pass again). With xprefix (the
+ * XCD-aware order, run_queue_xcd), the items of those jobs come from the 8 XCD queues (tickets
+ * [0, 8)) instead of the one queue (tickets[0]). */
 __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__ jobs,
                                                     const int64_t* __restrict__ prefix, int njobs, int64_t nitems,
                                                     int chunk, const uint64_t* __restrict__ mask,
@@ -149,7 +188,8 @@ __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__
                                                     const uint64_t* __restrict__ job_first,
                                                     const uint64_t* __restrict__ need, uint64_t cap_v,
                                                     uint64_t cap_t, float* __restrict__ pts, float* __restrict__ nrm,
-                                                    uint32_t* __restrict__ tris, uint64_t* __restrict__ ticket)
+                                                    uint32_t* __restrict__ tris, uint64_t* __restrict__ tickets,
+                                                    const int64_t* __restrict__ xprefix, int nemit)
 {
     __shared__ uint64_t s_case[256];
     __shared__ EmitRowsLds s_w[4];
@@ -157,7 +197,7 @@ __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__
     // triangle of the first job not emitted)
     if (need[0] > cap_v || need[1] > cap_t) return;
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
-    run_queue(ticket, prefix, njobs, nitems, chunk, [&](int j, int64_t blk) {
+    auto emit = [&](int j, int64_t blk) {
         const BatchJob& J = jobs[j];
         const uint64_t vbias = job_first[4 * j];
         if (J.unit)
@@ -168,7 +208,13 @@ __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__
             fe_emit_rows_block<uint32_t, false>(s_case, s_w, J.vol, mask + J.mask_base, J.g, J.iso,
                                                 counts + J.row_base, voff + J.row_base, toff + J.row_base, pts, nrm,
                                                 tris, vbias, blk);
-    });
+    };
+    if (xprefix)
+        run_queue_xcd(tickets, xprefix, (int64_t)njobs + 1, nemit, chunk, [&](int j, int64_t b) {
+            emit(j, emit_block_order(b, prefix[j + 1] - prefix[j], jobs[j].g));
+        });
+    else
+        run_queue(tickets, prefix, njobs, nitems, chunk, emit);
 }
 
 }  // namespace miso

@@ -65,8 +65,13 @@ def test_batch_matches_single_volume():
     assert tt == sum(r.triangles.shape[0] for r in refs)
     for shape in [(1, 1), (8, 16), (2, 4)]:
         b.set_tuning(*shape)
-        b.run()
-        _check(b, refs)
+        for order in ("xcd", "dispatch"):
+            b.set_queue_order(order)
+            assert b.queue_order() == order
+            b.run()
+            _check(b, refs)
+            b.run()          # the single graph replayed with this order
+            _check(b, refs)
     assert b.time_ms() > 0.0
     b.close()
 
@@ -119,6 +124,7 @@ def test_batch_autotune_keeps_results():
     best = b.autotune(reps=1)
     assert best > 0.0
     assert b.tuning() in [(x, c) for x in (2, 4, 8) for c in (1, 4, 16)]
+    assert b.queue_order() in ("dispatch", "xcd")
     b.run()
     _check(b, refs)
 
@@ -195,7 +201,8 @@ def test_split_phase_emit_twice_after_one_count(oracle_mod):
         return pts, nrm, tri
 
     fv, ft = ctypes.c_uint64(), ctypes.c_uint64()
-    for k in (3, len(jobs), len(jobs)):
+    for k, order in ((3, 0), (len(jobs), 0), (len(jobs), 1), (5, 1)):
+        check(L.miso_batch_set_queue_order(h, order), "queue order")
         pts, nrm, tri = emit(k)
         for q in range(k):
             check(L.miso_batch_job_offsets(h, q, ctypes.byref(fv), ctypes.byref(ft)), "offsets")
