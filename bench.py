@@ -227,6 +227,34 @@ def summarize_trace(trace_dir, out_path, command, title):
     print("\n".join(lines))
 
 
+def summarize_counters(pmc_dir, out_path, command, title):
+    """Per-kernel averages of a `rocprofv3 --pmc ...` run of this benchmark (miso kernels only),
+    with two derived figures where their counters are present: the texture addresser's busy
+    fraction (TA_BUSY_avr / GRBM_GUI_ACTIVE) and the L1's average read latency to the L2 in
+    cycles (TCP_TCC_READ_REQ_LATENCY_sum / TCP_TCC_READ_REQ_sum)."""
+    per = {}
+    for r in _csv_rows(pmc_dir, "counter_collection.csv"):
+        name = r["Kernel_Name"]
+        if "miso::" not in name:
+            continue
+        key = name.split("(")[0].replace("void ", "")
+        d = per.setdefault(key, {})
+        d.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    lines = [f"# {title}", f"# command: {command}",
+             "# per kernel, the average per dispatch of each counter; derived: ta_busy_frac = TA_BUSY_avr / "
+             "GRBM_GUI_ACTIVE, l2_read_latency = TCP_TCC_READ_REQ_LATENCY_sum / TCP_TCC_READ_REQ_sum (cycles)"]
+    for key, d in sorted(per.items()):
+        avg = {c: sum(v) / len(v) for c, v in d.items()}
+        parts = [f"{c}={avg[c]:.0f}" for c in sorted(avg)]
+        if "TA_BUSY_avr" in avg and avg.get("GRBM_GUI_ACTIVE"):
+            parts.append(f"ta_busy_frac={avg['TA_BUSY_avr'] / avg['GRBM_GUI_ACTIVE']:.3f}")
+        if avg.get("TCP_TCC_READ_REQ_sum") and "TCP_TCC_READ_REQ_LATENCY_sum" in avg:
+            parts.append(f"l2_read_latency={avg['TCP_TCC_READ_REQ_LATENCY_sum'] / avg['TCP_TCC_READ_REQ_sum']:.0f}")
+        lines.append(f"{key}: " + " ".join(parts))
+    open(out_path, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary, or None."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -609,13 +637,18 @@ def main():
     ap.add_argument("--pmc-command", default="", help="the profiled command, recorded in the summary")
     ap.add_argument("--summarize-trace", nargs=2, metavar=("TRACE_DIR", "OUT"),
                     help="write the per-kernel summary of a rocprofv3 --kernel-trace run and exit")
-    ap.add_argument("--trace-title", default="", help="first line of the --summarize-trace summary")
+    ap.add_argument("--trace-title", default="", help="first line of the --summarize-trace/-counters summary")
+    ap.add_argument("--summarize-counters", nargs=2, metavar=("PMC_DIR", "OUT"),
+                    help="write the per-kernel counter averages of a rocprofv3 --pmc run and exit")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc, command=args.pmc_command)
         return
     if args.summarize_trace:
         summarize_trace(*args.summarize_trace, command=args.pmc_command, title=args.trace_title)
+        return
+    if args.summarize_counters:
+        summarize_counters(*args.summarize_counters, command=args.pmc_command, title=args.trace_title)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the engine has no multi-GPU path (volumes would shard by z-slabs)",
