@@ -504,23 +504,32 @@ def bench_batched(reps, cpu_threads=None):
         batch_s = (time.perf_counter() - t0) / reps
         V, T = b.totals()
         same = compare(plain_meshes, b)
-        # pass 4's queue order at the tuned shape: one queue, or one per XCD
-        chosen, by_order = b.queue_order(), {}
-        for order in ("dispatch", "xcd"):
-            b.set_queue_order(order)
+        def wall_ms():
             b.run()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
                 b.run()
             torch.cuda.synchronize()
-            by_order[order] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+            return round((time.perf_counter() - t0) / reps * 1e3, 3)
+
+        # at the tuned shape: pass 4's queue order (one queue, or one per XCD), then the launch
+        # (six kernels from one graph, or one cooperative kernel)
+        chosen, by_order = b.queue_order(), {}
+        for order in ("dispatch", "xcd"):
+            b.set_queue_order(order)
+            by_order[order] = wall_ms()
         b.set_queue_order(chosen)
+        launch, by_launch = b.launch(), {}
+        for mode in ("graphs", "fused"):
+            b.set_launch(mode)
+            by_launch[mode] = wall_ms()
+        b.set_launch(launch)
         out = {"jobs": len(jobs), "plain_ms": round(plain_s * 1e3, 3), "batched_ms": round(batch_s * 1e3, 3),
                "batched_device_ms": round(b.time_ms(), 3), "speedup": round(plain_s / batch_s, 2),
                "plain_jobs_per_s": round(len(jobs) / plain_s, 1), "batched_jobs_per_s": round(len(jobs) / batch_s, 1),
                "nverts": V, "ntris": T, "tuning": list(b.tuning()), "queue_order": chosen,
-               "ms_by_queue_order": by_order,
+               "ms_by_queue_order": by_order, "launch": launch, "ms_by_launch": by_launch,
                "autotune_best_ms": round(tune_ms, 3),
                "identical_output": same}
         b.close()

@@ -195,8 +195,18 @@ int miso_batch_get_tuning(const miso_batch* b, int* blocks_per_cu, int* chunk);
 #define MISO_BATCH_QUEUE_XCD 1
 int miso_batch_set_queue_order(miso_batch* b, int order);
 int miso_batch_get_queue_order(const miso_batch* b, int* order);
+/* How miso_batch_run launches once the output capacity is known from an earlier run. GRAPHS (the
+ * default): one captured graph of six kernels (clear, passes 1, 2, 3, per-job counts, 4). FUSED:
+ * one cooperative kernel whose resident blocks run the same phases with a grid-wide barrier
+ * between them (at most the blocks per CU that fit at once; MISO_ERR_ARG where the device has no
+ * cooperative launch). Same output; a run whose output outgrows the buffers ends with pass 4
+ * from the graph either way. */
+#define MISO_BATCH_LAUNCH_GRAPHS 0
+#define MISO_BATCH_LAUNCH_FUSED 1
+int miso_batch_set_launch(miso_batch* b, int mode);
+int miso_batch_get_launch(const miso_batch* b, int* mode);
 /* times every setting of a small grid on this batch (reps runs each) -- resident blocks per CU,
- * items per ticket, pass 4's queue order -- and keeps the fastest */
+ * items per ticket, pass 4's queue order, launch mode -- and keeps the fastest */
 int miso_batch_autotune(miso_batch* b, int reps, float* best_ms);
 /* device time of the last run, passes 1-4 */
 int miso_batch_time_ms(const miso_batch* b, float* ms);

@@ -90,6 +90,8 @@ def lib():
         "miso_batch_get_tuning": [P, ctypes.POINTER(I), ctypes.POINTER(I)],
         "miso_batch_set_queue_order": [P, I],
         "miso_batch_get_queue_order": [P, ctypes.POINTER(I)],
+        "miso_batch_set_launch": [P, I],
+        "miso_batch_get_launch": [P, ctypes.POINTER(I)],
         "miso_batch_autotune": [P, I, ctypes.POINTER(F)],
         "miso_batch_time_ms": [P, ctypes.POINTER(F)],
         "miso_batch_create_empty": [P, PP],

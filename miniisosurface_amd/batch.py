@@ -87,6 +87,16 @@ class BatchExtraction:
         check(lib().miso_batch_get_queue_order(self._h, ctypes.byref(o)), "get_queue_order")
         return ("dispatch", "xcd")[o.value]
 
+    def set_launch(self, mode):
+        """How run() launches once the output size is known: "graphs" (six kernels replayed from
+        one HIP graph) or "fused" (one cooperative kernel, a grid barrier between the passes)."""
+        check(lib().miso_batch_set_launch(self._h, {"graphs": 0, "fused": 1}[mode]), "set_launch")
+
+    def launch(self):
+        o = ctypes.c_int()
+        check(lib().miso_batch_get_launch(self._h, ctypes.byref(o)), "get_launch")
+        return ("graphs", "fused")[o.value]
+
     def autotune(self, reps=3):
         ms = ctypes.c_float()
         check(lib().miso_batch_autotune(self._h, int(reps), ctypes.byref(ms)), "autotune")

@@ -37,6 +37,8 @@ struct miso_batch {
     uint64_t cap_v = 0, cap_t = 0;
     int blocks_per_cu = 4, chunk = 4, num_cus = 256;
     int queue_order = MISO_BATCH_QUEUE_DISPATCH;     // pass 4's work queue: one, or one per XCD
+    int launch = MISO_BATCH_LAUNCH_GRAPHS;           // graph of six kernels, or k_batch_fused
+    int fused_per_cu = 0;                            // k_batch_fused blocks resident per CU (0: none)
     std::vector<int64_t> xcd_prefix;                 // 8 x (njobs + 1): XCD x's pass 4 items before job j
     int64_t* d_xcd_prefix = nullptr;
     int64_t cap_xcd = 0;
@@ -131,7 +133,7 @@ int enqueue_emit_into(miso_batch* b, int64_t nemit, float* pts, float* nrm, uint
     hipLaunchKernelGGL(k_batch_emit, dim3(grid), dim3(256), 0, b->stream, (const BatchJob*)b->d_jobs,
                        (const int64_t*)b->d_lane_prefix, b->njobs, nlane, b->chunk, (const uint64_t*)b->d_mask,
                        (const RowCounts*)b->d_counts, (const uint64_t*)b->d_voff, (const uint64_t*)b->d_toff,
-                       (const uint64_t*)b->d_jobout, need, cap_v, cap_t, pts, nrm, tris, tk,
+                       need, cap_v, cap_t, pts, nrm, tris, tk,
                        xcd ? (const int64_t*)b->d_xcd_prefix : (const int64_t*)nullptr, (int)nemit);
     HIPCALL(hipGetLastError());
     return MISO_OK;
@@ -229,6 +231,10 @@ int batch_setup(miso_batch* b, void* user_stream)
     int dev = 0;
     HIPCALL(hipGetDevice(&dev));
     HIPCALL(hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int coop = 0;
+    HIPCALL(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (coop)
+        HIPCALL(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->fused_per_cu, k_batch_fused, 256, 0));
     HIPCALL(hipEventCreate(&b->ev0));
     HIPCALL(hipEventCreate(&b->ev1));
     return MISO_OK;
@@ -350,6 +356,39 @@ int batch_init(miso_batch* b, const miso_volume* vols, const float* isovals, int
     return capture(b, &b->g_scan, [&] { return enqueue_scan(b); });
 }
 
+/* Passes 1-4 as one cooperative launch of k_batch_fused (all blocks resident at once) after a
+ * small kernel clears its tickets and scan state (cheaper there than before a grid barrier that
+ * every block reaches at once), then the per-job counts and totals to the host. */
+int enqueue_fused(miso_batch* b)
+{
+    const bool xcd = b->queue_order == MISO_BATCH_QUEUE_XCD;
+    int64_t grid = (int64_t)b->num_cus * std::min(b->blocks_per_cu, b->fused_per_cu);
+    if (xcd) grid = grid / 8 * 8;   // every XCD queue has blocks
+    if (grid < 8) return fail(MISO_ERR_ARG, "fused launch: %lld resident blocks", (long long)grid);
+    const BatchJob* jobs = b->d_jobs;
+    const int* group_job = b->d_group_job;
+    const int64_t* cls_prefix = b->d_cls_prefix;
+    const int64_t* lane_prefix = b->d_lane_prefix;
+    const int64_t* xprefix = xcd ? b->d_xcd_prefix : nullptr;
+    int ngroups = b->ngroups, njobs = b->njobs, chunk = b->chunk;
+    int64_t ncls = b->cls_prefix.back(), nlane = b->lane_prefix.back();
+    int64_t R = b->R, ntiles = b->ntiles;
+    uint64_t cap_v = b->cap_v, cap_t = b->cap_t;
+    void* args[] = {&jobs,      &group_job, &cls_prefix, &ngroups,     &ncls,  &lane_prefix, &njobs,
+                    &nlane,     &chunk,     &b->d_mask,  &b->d_counts, &b->d_trims, &b->d_voff, &b->d_toff,
+                    &b->d_state, &R,        &ntiles,     &b->d_jobout, &cap_v, &cap_t,     &b->d_pts,
+                    &b->d_nrm,  &b->d_tris, &xprefix};
+    hipLaunchKernelGGL(k_batch_zero, dim3((unsigned)std::min<int64_t>(1024, (b->state_words + 255) / 256)), dim3(256),
+                       0, b->stream, b->d_state, b->state_words);
+    HIPCALL(hipLaunchCooperativeKernel((const void*)k_batch_fused, dim3((unsigned)grid), dim3(256), args, 0,
+                                       b->stream));
+    HIPCALL(hipMemcpyAsync(b->h_jobout, b->d_jobout, 4 * (size_t)b->njobs * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIPCALL(hipMemcpyAsync(b->h_jobout + 4 * b->njobs, b->totals(), 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           b->stream));
+    return MISO_OK;
+}
+
 int batch_run(miso_batch* b)
 {
     if (b->njobs == 0) return fail(MISO_ERR_STATE, "batch has no jobs");
@@ -357,7 +396,15 @@ int batch_run(miso_batch* b)
         if (int rc = capture(b, &b->g_scan, [&] { return enqueue_scan(b); })) return rc;
     HIPCALL(hipEventRecord(b->ev0, b->stream));
     const uint64_t* tot = b->h_jobout + 4 * b->njobs;
-    if (b->ran && b->g_all) {
+    if (b->ran && b->d_pts && b->g_emit && b->launch == MISO_BATCH_LAUNCH_FUSED) {
+        // as the graph below: pass 4 writes nothing if the totals outgrew the buffers
+        if (int rc = enqueue_fused(b)) return rc;
+        HIPCALL(hipStreamSynchronize(b->stream));
+        if (tot[0] > b->cap_v || tot[1] > b->cap_t) {
+            if (int rc = batch_reserve(b, tot[0], tot[1])) return rc;
+            HIPCALL(hipGraphLaunch(b->g_emit, b->stream));
+        }
+    } else if (b->ran && b->g_all) {
         // capacity known from the last run: all four passes in one graph; the emit pass writes
         // nothing if the totals outgrew the buffers, and is then run again below
         HIPCALL(hipGraphLaunch(b->g_all, b->stream));
@@ -495,6 +542,24 @@ extern "C" int miso_batch_get_queue_order(const miso_batch* b, int* order)
     return MISO_OK;
 }
 
+extern "C" int miso_batch_set_launch(miso_batch* b, int mode)
+{
+    if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
+    if (mode != MISO_BATCH_LAUNCH_GRAPHS && mode != MISO_BATCH_LAUNCH_FUSED)
+        return fail(MISO_ERR_ARG, "launch mode is MISO_BATCH_LAUNCH_GRAPHS or MISO_BATCH_LAUNCH_FUSED");
+    if (mode == MISO_BATCH_LAUNCH_FUSED && b->fused_per_cu < 1)
+        return fail(MISO_ERR_ARG, "the device has no cooperative launch for the fused kernel");
+    b->launch = mode;
+    return MISO_OK;
+}
+
+extern "C" int miso_batch_get_launch(const miso_batch* b, int* mode)
+{
+    if (!b || !mode) return fail(MISO_ERR_ARG, "NULL argument");
+    *mode = b->launch;
+    return MISO_OK;
+}
+
 extern "C" int miso_batch_get_tuning(const miso_batch* b, int* blocks_per_cu, int* chunk)
 {
     if (!b) return fail(MISO_ERR_ARG, "batch is NULL");
@@ -510,29 +575,35 @@ extern "C" int miso_batch_autotune(miso_batch* b, int reps, float* best_ms)
     static const int kBlocks[] = {2, 4, 8};
     static const int kChunk[] = {1, 4, 16};
     static const int kOrder[] = {MISO_BATCH_QUEUE_DISPATCH, MISO_BATCH_QUEUE_XCD};
+    static const int kLaunch[] = {MISO_BATCH_LAUNCH_GRAPHS, MISO_BATCH_LAUNCH_FUSED};
     float best = -1.0f;
-    int bb = b->blocks_per_cu, bc = b->chunk, bo = b->queue_order;
-    for (int o : kOrder)
-        for (int x : kBlocks)
-            for (int c : kChunk) {
-                if (int rc = miso_batch_set_tuning(b, x, c)) return rc;
-                if (int rc = miso_batch_set_queue_order(b, o)) return rc;
-                if (int rc = batch_run(b)) return rc;   // warm: sizes the output, captures the graphs
-                float sum = 0.0f;
-                for (int r = 0; r < reps; ++r) {
-                    if (int rc = batch_run(b)) return rc;
-                    sum += b->last_ms;
+    int bb = b->blocks_per_cu, bc = b->chunk, bo = b->queue_order, bl = b->launch;
+    for (int l : kLaunch)
+        for (int o : kOrder)
+            for (int x : kBlocks)
+                for (int c : kChunk) {
+                    if (l == MISO_BATCH_LAUNCH_FUSED && (b->fused_per_cu < 1 || x > b->fused_per_cu)) continue;
+                    if (int rc = miso_batch_set_tuning(b, x, c)) return rc;
+                    if (int rc = miso_batch_set_queue_order(b, o)) return rc;
+                    if (int rc = miso_batch_set_launch(b, l)) return rc;
+                    if (int rc = batch_run(b)) return rc;   // warm: sizes the output, captures the graphs
+                    float sum = 0.0f;
+                    for (int r = 0; r < reps; ++r) {
+                        if (int rc = batch_run(b)) return rc;
+                        sum += b->last_ms;
+                    }
+                    const float ms = sum / reps;
+                    if (best < 0.0f || ms < best) {
+                        best = ms;
+                        bb = x;
+                        bc = c;
+                        bo = o;
+                        bl = l;
+                    }
                 }
-                const float ms = sum / reps;
-                if (best < 0.0f || ms < best) {
-                    best = ms;
-                    bb = x;
-                    bc = c;
-                    bo = o;
-                }
-            }
     if (int rc = miso_batch_set_tuning(b, bb, bc)) return rc;
     if (int rc = miso_batch_set_queue_order(b, bo)) return rc;
+    if (int rc = miso_batch_set_launch(b, bl)) return rc;
     if (best_ms) *best_ms = best;
     return MISO_OK;
 }

@@ -16,6 +16,8 @@
 #ifndef MINIISO_FE_BATCH_H
 #define MINIISO_FE_BATCH_H
 
+#include <hip/hip_cooperative_groups.h>
+
 #include "fe_rows.h"
 
 namespace miso {
@@ -123,25 +125,32 @@ __global__ __launch_bounds__(256) void k_batch_zero(uint64_t* __restrict__ words
 /* Pass 1 over groups of jobs: group q is jobs [group_job[q], group_job[q + 1]), consecutive jobs
  * that read the same volume with the same dimensions (one volume at several isovalues); an item
  * loads a block of its rows once and writes every job's sign-mask words. */
+/* Pass 1 item blk of classify group q (the jobs [group_job[q], group_job[q + 1]) on one volume). */
+__device__ __forceinline__ void batch_classify_item(const BatchJob* __restrict__ jobs,
+                                                    const int* __restrict__ group_job, uint64_t* __restrict__ mask,
+                                                    int q, int64_t blk)
+{
+    const int j0 = group_job[q], n = group_job[q + 1] - j0;
+    const BatchJob& J = jobs[j0];
+    auto sink = [&](int t, float& iso, uint64_t*& m) {
+        iso = jobs[j0 + t].iso;
+        m = mask + jobs[j0 + t].mask_base;
+    };
+    // an item is 4 kShortRows rows of a short-row job (batch_classify_rows), else 4 rows
+    if (J.g.W <= 4)
+        fe_classify_short_rows_n(J.vol, J.g, n, sink, blk);
+    else
+        fe_classify_block_n(J.vol, J.g, n, sink, blk);
+}
+
 __global__ __launch_bounds__(256) void k_batch_classify(const BatchJob* __restrict__ jobs,
                                                         const int* __restrict__ group_job,
                                                         const int64_t* __restrict__ prefix, int ngroups,
                                                         int64_t nitems, int chunk, uint64_t* __restrict__ mask,
                                                         uint64_t* __restrict__ ticket)
 {
-    run_queue(ticket, prefix, ngroups, nitems, chunk, [&](int q, int64_t blk) {
-        const int j0 = group_job[q], n = group_job[q + 1] - j0;
-        const BatchJob& J = jobs[j0];
-        auto sink = [&](int t, float& iso, uint64_t*& m) {
-            iso = jobs[j0 + t].iso;
-            m = mask + jobs[j0 + t].mask_base;
-        };
-        // an item is 4 kShortRows rows of a short-row job (batch_classify_rows), else 4 rows
-        if (J.g.W <= 4)
-            fe_classify_short_rows_n(J.vol, J.g, n, sink, blk);
-        else
-            fe_classify_block_n(J.vol, J.g, n, sink, blk);
-    });
+    run_queue(ticket, prefix, ngroups, nitems, chunk,
+              [&](int q, int64_t blk) { batch_classify_item(jobs, group_job, mask, q, blk); });
 }
 
 __global__ __launch_bounds__(256) void k_batch_count(const BatchJob* __restrict__ jobs,
@@ -156,15 +165,13 @@ __global__ __launch_bounds__(256) void k_batch_count(const BatchJob* __restrict_
     });
 }
 
-/* Per job: first vertex, first triangle, vertex and triangle counts (from the scan). */
-__global__ __launch_bounds__(256) void k_batch_job_counts(const BatchJob* __restrict__ jobs, int njobs, int64_t R,
-                                                          const uint64_t* __restrict__ voff,
-                                                          const uint64_t* __restrict__ toff,
-                                                          const uint64_t* __restrict__ totals,
-                                                          uint64_t* __restrict__ out)
+/* Job j's first vertex, first triangle, vertex and triangle counts (from the scan). */
+__device__ __forceinline__ void batch_job_counts_one(const BatchJob* __restrict__ jobs, int njobs, int64_t R,
+                                                     const uint64_t* __restrict__ voff,
+                                                     const uint64_t* __restrict__ toff,
+                                                     const uint64_t* __restrict__ totals, uint64_t* __restrict__ out,
+                                                     int j)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= njobs) return;
     const int64_t b = jobs[j].row_base;
     const int64_t e = j + 1 < njobs ? jobs[j + 1].row_base : R;
     const uint64_t vb = b < R ? voff[b] : totals[0], tb = b < R ? toff[b] : totals[1];
@@ -175,17 +182,66 @@ __global__ __launch_bounds__(256) void k_batch_job_counts(const BatchJob* __rest
     out[4 * j + 3] = te - tb;
 }
 
+__global__ __launch_bounds__(256) void k_batch_job_counts(const BatchJob* __restrict__ jobs, int njobs, int64_t R,
+                                                          const uint64_t* __restrict__ voff,
+                                                          const uint64_t* __restrict__ toff,
+                                                          const uint64_t* __restrict__ totals,
+                                                          uint64_t* __restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs) batch_job_counts_one(jobs, njobs, R, voff, toff, totals, out, j);
+}
+
+/* Pass 4 of job j's row block blk into the shared output; the job's triangles index its own
+ * vertices, from its first one, voff at its first row. */
+__device__ __forceinline__ void batch_emit_item(const uint64_t* __restrict__ s_case, EmitRowsLds* s_w,
+                                                const BatchJob* __restrict__ jobs, const uint64_t* __restrict__ mask,
+                                                const RowCounts* __restrict__ counts,
+                                                const uint64_t* __restrict__ voff, const uint64_t* __restrict__ toff,
+                                                float* __restrict__ pts, float* __restrict__ nrm,
+                                                uint32_t* __restrict__ tris, int j, int64_t blk)
+{
+    const BatchJob& J = jobs[j];
+    const uint64_t vbias = voff[J.row_base];
+    if (J.unit)
+        fe_emit_rows_block<uint32_t, true>(s_case, s_w, J.vol, mask + J.mask_base, J.g, J.iso, counts + J.row_base,
+                                           voff + J.row_base, toff + J.row_base, pts, nrm, tris, vbias, blk);
+    else
+        fe_emit_rows_block<uint32_t, false>(s_case, s_w, J.vol, mask + J.mask_base, J.g, J.iso, counts + J.row_base,
+                                            voff + J.row_base, toff + J.row_base, pts, nrm, tris, vbias, blk);
+}
+
+/* Pass 4 for the first nemit jobs, from the one queue (items [0, nitems), tickets[0]) or, with
+ * xprefix, the XCD queues (run_queue_xcd, tickets[0 .. 8)). */
+__device__ __forceinline__ void batch_emit_queue(const uint64_t* __restrict__ s_case, EmitRowsLds* s_w,
+                                                 const BatchJob* __restrict__ jobs, const int64_t* __restrict__ prefix,
+                                                 int njobs, int64_t nitems, int chunk,
+                                                 const uint64_t* __restrict__ mask,
+                                                 const RowCounts* __restrict__ counts,
+                                                 const uint64_t* __restrict__ voff, const uint64_t* __restrict__ toff,
+                                                 float* __restrict__ pts, float* __restrict__ nrm,
+                                                 uint32_t* __restrict__ tris, uint64_t* __restrict__ tickets,
+                                                 const int64_t* __restrict__ xprefix, int nemit)
+{
+    if (xprefix)
+        run_queue_xcd(tickets, xprefix, (int64_t)njobs + 1, nemit, chunk, [&](int j, int64_t b) {
+            batch_emit_item(s_case, s_w, jobs, mask, counts, voff, toff, pts, nrm, tris, j,
+                            emit_block_order(b, prefix[j + 1] - prefix[j], jobs[j].g));
+        });
+    else
+        run_queue(tickets, prefix, njobs, nitems, chunk, [&](int j, int64_t blk) {
+            batch_emit_item(s_case, s_w, jobs, mask, counts, voff, toff, pts, nrm, tris, j, blk);
+        });
+}
+
 /* Pass 4 for the first jobs (items [0, nitems)); writes nothing when their output exceeds the
- * capacity (the host then grows the buffers and runs this pass again). With xprefix (the
- * XCD-aware order, run_queue_xcd), the items of those jobs come from the 8 XCD queues (tickets
- * [0, 8)) instead of the one queue (tickets[0]). */
+ * capacity (the host then grows the buffers and runs this pass again). */
 __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__ jobs,
                                                     const int64_t* __restrict__ prefix, int njobs, int64_t nitems,
                                                     int chunk, const uint64_t* __restrict__ mask,
                                                     const RowCounts* __restrict__ counts,
                                                     const uint64_t* __restrict__ voff,
                                                     const uint64_t* __restrict__ toff,
-                                                    const uint64_t* __restrict__ job_first,
                                                     const uint64_t* __restrict__ need, uint64_t cap_v,
                                                     uint64_t cap_t, float* __restrict__ pts, float* __restrict__ nrm,
                                                     uint32_t* __restrict__ tris, uint64_t* __restrict__ tickets,
@@ -197,24 +253,57 @@ __global__ __launch_bounds__(256) void k_batch_emit(const BatchJob* __restrict__
     // triangle of the first job not emitted)
     if (need[0] > cap_v || need[1] > cap_t) return;
     s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
-    auto emit = [&](int j, int64_t blk) {
+    batch_emit_queue(s_case, s_w, jobs, prefix, njobs, nitems, chunk, mask, counts, voff, toff, pts, nrm, tris,
+                     tickets, xprefix, nemit);
+}
+
+/* Everything batch_run does on the device after clearing the tickets and the scan state, in one
+ * cooperative launch: the grid (every block resident at once) runs pass 1 over the classify
+ * groups, pass 2, the scan (tiles taken in order from its ticket) and pass 4 (when the totals
+ * fit the output), with a grid-wide barrier between the phases, which also orders every block's
+ * writes before the next phase's reads; the per-job counts for the host are written beside
+ * pass 4, which does not read them. The queues and block functions are those of the separate
+ * kernels, so the output is theirs. state: [0..2] tickets, [3..4] totals, [5..12] the XCD
+ * tickets, [13..] the scan state (its ticket, then ntiles vertex and ntiles triangle tile
+ * status words). */
+__global__ __launch_bounds__(256) void k_batch_fused(const BatchJob* __restrict__ jobs,
+                                                     const int* __restrict__ group_job,
+                                                     const int64_t* __restrict__ cls_prefix, int ngroups, int64_t ncls,
+                                                     const int64_t* __restrict__ lane_prefix, int njobs,
+                                                     int64_t nlane, int chunk, uint64_t* __restrict__ mask,
+                                                     RowCounts* __restrict__ counts, RowTrim* __restrict__ trims,
+                                                     uint64_t* __restrict__ voff, uint64_t* __restrict__ toff,
+                                                     uint64_t* __restrict__ state, int64_t R, int64_t ntiles,
+                                                     uint64_t* __restrict__ jobout, uint64_t cap_v, uint64_t cap_t,
+                                                     float* __restrict__ pts, float* __restrict__ nrm,
+                                                     uint32_t* __restrict__ tris, const int64_t* __restrict__ xprefix)
+{
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    __shared__ uint64_t s_case[256];
+    __shared__ EmitRowsLds s_w[4];
+    s_case[threadIdx.x] = kCaseEdges[threadIdx.x];
+    run_queue(state + 0, cls_prefix, ngroups, ncls, chunk,
+              [&](int q, int64_t blk) { batch_classify_item(jobs, group_job, mask, q, blk); });
+    grid.sync();
+    run_queue(state + 1, lane_prefix, njobs, nlane, chunk, [&](int j, int64_t blk) {
         const BatchJob& J = jobs[j];
-        const uint64_t vbias = job_first[4 * j];
-        if (J.unit)
-            fe_emit_rows_block<uint32_t, true>(s_case, s_w, J.vol, mask + J.mask_base, J.g, J.iso,
-                                               counts + J.row_base, voff + J.row_base, toff + J.row_base, pts, nrm,
-                                               tris, vbias, blk);
-        else
-            fe_emit_rows_block<uint32_t, false>(s_case, s_w, J.vol, mask + J.mask_base, J.g, J.iso,
-                                                counts + J.row_base, voff + J.row_base, toff + J.row_base, pts, nrm,
-                                                tris, vbias, blk);
-    };
-    if (xprefix)
-        run_queue_xcd(tickets, xprefix, (int64_t)njobs + 1, nemit, chunk, [&](int j, int64_t b) {
-            emit(j, emit_block_order(b, prefix[j + 1] - prefix[j], jobs[j].g));
-        });
-    else
-        run_queue(tickets, prefix, njobs, nitems, chunk, emit);
+        fe_count_rows_block(mask + J.mask_base, J.g, counts + J.row_base, trims + J.row_base, blk);
+    });
+    grid.sync();
+    uint64_t* scan = state + 13;
+    uint64_t* totals = state + 3;
+    for (;;) {
+        const int64_t tile = fe_scan_ticket(scan);
+        if (tile >= ntiles) break;
+        fe_scan_tile(counts, R, voff, toff, scan, ntiles, totals, tile);
+    }
+    grid.sync();
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < njobs; j += (int64_t)gridDim.x * blockDim.x)
+        batch_job_counts_one(jobs, njobs, R, voff, toff, totals, jobout, (int)j);
+    if (totals[0] > cap_v || totals[1] > cap_t) return;
+    batch_emit_queue(s_case, s_w, jobs, lane_prefix, njobs, nlane, chunk, mask, counts, voff, toff, pts, nrm, tris,
+                     xprefix ? state + 5 : state + 2, xprefix, njobs);
 }
 
 }  // namespace miso

@@ -288,20 +288,15 @@ __device__ uint64_t lookback(uint64_t* __restrict__ status, int64_t tile)
     }
 }
 
-__global__ __launch_bounds__(256) void k_fe_scan(const RowCounts* __restrict__ counts, int64_t R,
-                                                 uint64_t* __restrict__ voff, uint64_t* __restrict__ toff,
-                                                 uint64_t* __restrict__ state, int64_t ntiles,
-                                                 uint64_t* __restrict__ totals)
+/* One 4096-row tile of the scan (tile numbers are taken from state[0] in launch order, so that a
+ * tile's predecessors have started: the look-back waits for them). */
+__device__ __forceinline__ void fe_scan_tile(const RowCounts* __restrict__ counts, int64_t R,
+                                             uint64_t* __restrict__ voff, uint64_t* __restrict__ toff,
+                                             uint64_t* __restrict__ state, int64_t ntiles,
+                                             uint64_t* __restrict__ totals, int64_t tile)
 {
-    __shared__ int64_t s_tile;
     __shared__ uint64_t s_wv[4], s_wt[4];
     __shared__ uint64_t s_ev, s_et;
-    // state[0]: ticket, state[1 .. ntiles]: vertex status, state[1+ntiles ..]: triangle status
-    if (threadIdx.x == 0)
-        s_tile = (int64_t)__hip_atomic_fetch_add((unsigned long long*)&state[0], 1ull, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const int64_t tile = s_tile;
     uint64_t* stv = state + 1;
     uint64_t* stt = state + 1 + ntiles;
     const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -378,6 +373,27 @@ __global__ __launch_bounds__(256) void k_fe_scan(const RowCounts* __restrict__ c
         pv += v[u];
         pt += t[u];
     }
+}
+
+/* The next tile number of the scan, for every thread of the block. */
+__device__ __forceinline__ int64_t fe_scan_ticket(uint64_t* __restrict__ state)
+{
+    __shared__ int64_t s_tile;
+    __syncthreads();   // every thread has read the previous tile number
+    if (threadIdx.x == 0)
+        s_tile = (int64_t)__hip_atomic_fetch_add((unsigned long long*)&state[0], 1ull, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return s_tile;
+}
+
+__global__ __launch_bounds__(256) void k_fe_scan(const RowCounts* __restrict__ counts, int64_t R,
+                                                 uint64_t* __restrict__ voff, uint64_t* __restrict__ toff,
+                                                 uint64_t* __restrict__ state, int64_t ntiles,
+                                                 uint64_t* __restrict__ totals)
+{
+    // state[0]: ticket, state[1 .. ntiles]: vertex status, state[1+ntiles ..]: triangle status
+    fe_scan_tile(counts, R, voff, toff, state, ntiles, totals, fe_scan_ticket(state));
 }
 
 /* ------------------------------------------------------------------ pass 4: emit */

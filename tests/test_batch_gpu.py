@@ -76,6 +76,55 @@ def test_batch_matches_single_volume():
     b.close()
 
 
+def test_batch_fused_launch_matches():
+    """The fused launch (one cooperative kernel for passes 1-4) gives every job the same mesh as
+    the graph launch, on each replay, at each launch shape and queue order."""
+    from miniisosurface_amd.batch import BatchExtraction
+    jobs, tens = _jobs(seed=7)
+    print(f"[batch fused] {len(jobs)} jobs", flush=True)
+    refs = [_single(v, iso, sp, org) for _, iso, v, sp, org in jobs]
+    b = BatchExtraction([(dv, iso) for dv, iso, _, _, _ in jobs])
+    b.set_launch("fused")
+    assert b.launch() == "fused"
+    b.run()                       # first run sizes the output through the graphs
+    _check(b, refs)
+    for shape in [(4, 4), (1, 1), (2, 16), (8, 1)]:
+        b.set_tuning(*shape)
+        for order in ("dispatch", "xcd"):
+            b.set_queue_order(order)
+            print(f"  shape {shape} order {order}", flush=True)
+            for rep in range(2):
+                b.run()
+                _check(b, refs)
+    b.set_launch("graphs")
+    b.run()
+    _check(b, refs)
+    b.close()
+
+
+def test_batch_fused_content_change_regrows_output():
+    """A fused replay whose totals outgrow the output writes nothing in its pass 4; the run then
+    grows the buffers and runs pass 4 again from the graph."""
+    import torch
+    from miniisosurface_amd import DeviceVolume
+    from miniisosurface_amd.batch import BatchExtraction
+    t = torch.zeros((40, 50, 60), dtype=torch.float32, device="cuda")
+    t[10:20, 10:20, 10:20] = 1.0
+    b = BatchExtraction([(DeviceVolume.from_tensor(t), 0.5)])
+    b.set_launch("fused")
+    b.run()
+    b.run()
+    v1, t1 = b.totals()
+    t.copy_(torch.from_numpy(np.random.default_rng(1).uniform(0, 1, (40, 50, 60)).astype(np.float32)))
+    b.run()
+    v2, t2 = b.totals()
+    assert v2 > v1 and t2 > t1
+    _check(b, [_single(t.cpu().numpy(), 0.5, (1, 1, 1), (0, 0, 0))])
+    b.run()
+    _check(b, [_single(t.cpu().numpy(), 0.5, (1, 1, 1), (0, 0, 0))])
+    b.close()
+
+
 def test_batch_shared_volume_groups():
     """Consecutive jobs on one volume are classified together, at most 16 to a group (the rows
     loaded once for all of them): 20 isovalues of a volume with 300-voxel rows (16-byte loads and
@@ -125,6 +174,7 @@ def test_batch_autotune_keeps_results():
     assert best > 0.0
     assert b.tuning() in [(x, c) for x in (2, 4, 8) for c in (1, 4, 16)]
     assert b.queue_order() in ("dispatch", "xcd")
+    assert b.launch() in ("graphs", "fused")
     b.run()
     _check(b, refs)
 
