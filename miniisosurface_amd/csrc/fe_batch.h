@@ -11,7 +11,8 @@
  * grid rows of the group (16 when rows hold at most 256 voxels), loaded once for all its jobs.
  * Pass 3 is the single-volume scan over the concatenated rows, so job j's output is the
  * contiguous range [voff[row_base_j], voff[row_base_{j+1}]) of one shared buffer; its triangles
- * index its own vertices from 0.
+ * index its own vertices from 0. k_batch_fused runs the same queues and block functions for all
+ * four passes in one cooperative launch, with grid barriers between the passes.
  */
 #ifndef MINIISO_FE_BATCH_H
 #define MINIISO_FE_BATCH_H
