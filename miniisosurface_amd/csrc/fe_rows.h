@@ -155,20 +155,10 @@ __device__ __forceinline__ CornerWords corner_words(const LaneWords& L)
     return C;
 }
 
-/* marching-cubes case of the cell at bit b: corner bits v0..v7 (FlyingEdgesAlgorithm.cpp:138-157) */
-__device__ __forceinline__ int case_of(const CornerWords& C, int b)
-{
-    const int sh = b & 31;
-    const bool hi = b >= 32;
-    auto half = [&](uint64_t w) -> uint32_t { return hi ? (uint32_t)(w >> 32) : (uint32_t)w; };
-    const uint32_t c = ((half(C.m[0]) >> sh) & 1u) | ((half(C.s[0]) >> sh) & 1u) << 1 |
-                       ((half(C.s[1]) >> sh) & 1u) << 2 | ((half(C.m[1]) >> sh) & 1u) << 3 |
-                       ((half(C.m[2]) >> sh) & 1u) << 4 | ((half(C.s[2]) >> sh) & 1u) << 5 |
-                       ((half(C.s[3]) >> sh) & 1u) << 6 | ((half(C.m[3]) >> sh) & 1u) << 7;
-    return (int)c;
-}
-
-__device__ __forceinline__ int case_at(const LaneWords& L, int b) { return case_of(corner_words(L), b); }
+/* A cell's marching-cubes case has corner bit q = v_q, in the reference's order
+ * (FlyingEdgesAlgorithm.cpp:138-157): v0..v7 = m0 s0 s1 m1 m2 s2 s3 m3 of CornerWords, i.e. rows
+ * (j,k), (j+1,k), (j,k+1), (j+1,k+1) at x = i (m) and i + 1 (s), taken (i,j) (i+1,j) (i+1,j+1)
+ * (i,j+1) in plane k, then the same in plane k + 1. */
 
 __device__ __forceinline__ uint64_t active_cells(const CornerWords& C, uint64_t eb)
 {
@@ -190,7 +180,7 @@ __device__ __forceinline__ uint64_t active_cells(const LaneWords& L, uint64_t eb
  * masked out; cells with none contribute 0. */
 __device__ __forceinline__ uint32_t word_tri_count(const CornerWords& C, uint64_t eb)
 {
-    // corners v0..v7 of the cells (case_of's bit order)
+    // corners v0..v7 of the cells (the case's bit order)
     const uint64_t v0 = C.m[0], v1 = C.s[0], v2 = C.s[1], v3 = C.m[1];
     const uint64_t v4 = C.m[2], v5 = C.s[2], v6 = C.s[3], v7 = C.m[3];
     const uint64_t a01 = v0 & v1, a23 = v2 & v3, a45 = v4 & v5, a67 = v6 & v7;
@@ -492,7 +482,7 @@ struct EmitRowsLds {
 static_assert(sizeof(EmitRowsLds) == 8 * 64 * 8 + 8 * 64 * 4 + (kEmitCellCap + kEmitTriCap) * 4,
               "the staging window fits in the triangles' bytes");
 
-/* case of the cell at bit b of a 32-bit half (corner bits v0..v7 as case_of) */
+/* case of the cell at bit b of a 32-bit half: h[q] holds corner v_q's bits (the case's bit order) */
 __device__ __forceinline__ uint32_t case_of_half(const uint32_t h[8], int b)
 {
     uint32_t c = __builtin_amdgcn_ubfe(h[0], b, 1);
