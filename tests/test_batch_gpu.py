@@ -295,6 +295,29 @@ def test_batch_mixed_256_volumes_against_oracle(oracle_mod):
 
 
 @pytest.mark.slow
+def test_batch_mixed_256_fused_matches_graphs():
+    """The 256-volume workload (seed 0) with the fused cooperative launch, in both queue orders:
+    every job's mesh bit-identical to the graph launch's (which the oracle test above pins)."""
+    import torch
+
+    from miniisosurface_amd import DeviceVolume, synth
+    from miniisosurface_amd.batch import BatchExtraction
+    vols = synth.batch_volumes(256, seed=0, device="cuda")
+    b = BatchExtraction([(DeviceVolume.from_tensor(t), iso) for t, iso in vols])
+    b.set_launch("graphs")
+    b.run()
+    refs = [b.mesh(q) for q in range(len(vols))]
+    b.set_launch("fused")
+    for order in ("dispatch", "xcd"):
+        b.set_queue_order(order)
+        b.run()
+        torch.cuda.synchronize()
+        print(f"[batch 256 fused] order {order}, totals {b.totals()}", flush=True)
+        _check(b, refs)
+    b.close()
+
+
+@pytest.mark.slow
 def test_batch_gyroid_512_sixteen_isovalues_against_oracle(oracle_mod):
     """The 512^3 gyroid (with noise, seed 0) at 16 isovalues evenly spaced in [-1, 1] as one
     batch over the one resident volume; every job's mesh bit-exact against the oracle."""
