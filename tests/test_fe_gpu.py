@@ -84,6 +84,12 @@ def _cases():
         iso_vox[k, j, i] = 1.0
     yield "isolated_130x40x7", iso_vox, 0.5, (1, 1, 1), (0, 0, 0)
     yield "checker_64", (np.indices((64, 64, 64)).sum(0) % 2).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
+    # every edge cut on rows of 4096 voxels: one wavefront per row writes about 12,000 vertices
+    # (dozens of pass 4 staging windows) and 4,095 active cells per row (several cell rounds);
+    # and rows of 1000 voxels, where 4 rows share a wavefront
+    yield "checker_4096x4x3", (np.indices((3, 4, 4096)).sum(0) % 2).astype(np.float32), 0.5, (1, 1, 1), (0, 0, 0)
+    yield "checker_1000x9x5_sp", (np.indices((5, 9, 1000)).sum(0) % 2).astype(np.float32), 0.5, (0.7, 1.3, 2.0), \
+        (1.0, -1.0, 0.5)
     yield "gauss_96x80x72", synth.gaussians((96, 80, 72), 6, 3, sigma_range=(5.0, 14.0)), 0.3, (1, 1, 1), (0, 0, 0)
     yield "gyroid_128", synth.gyroid(128, 1), 0.0, (1, 1, 1), (0, 0, 0)
     yield "noise_128", synth.value_noise(128, 2), 0.5, (0.5, 0.5, 0.5), (1.0, 2.0, 3.0)
